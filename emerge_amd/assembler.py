@@ -1,0 +1,102 @@
+"""Drop-in assembler backed by the HIP kernels.
+
+``HipAssembler`` mirrors the reference ``Assembler`` plug-in
+(ASM/assembler.py:100-108, ``ASM/`` = src/emerge/_emerge/physics/microwave/
+assembly/): it is installed as ``model.mw.assembler = HipAssembler(settings)``
+(attribute set at physics/microwave/microwave_3d.py:128) and keeps the
+``cached_matrices`` semantics.  Its narrow entry point
+``tet_mass_stiffness_matrices(field, er, ur) -> (E, B)`` replaces
+ASM/curlcurl.py:145-173 with the same inputs (``(3,3,T)`` complex tensors)
+and outputs (scipy CSR, int32 indices, identical sparsity pattern).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.sparse import csr_matrix
+
+from ._lib import Context
+
+
+def isotropic_scalars(tensor: np.ndarray, name: str) -> np.ndarray:
+    """(3,3,T) tensor -> (T,) scalars; raises if any tet is anisotropic.
+
+    The GPU element kernel takes one complex scalar per tet for eps_r and
+    mu_r; anisotropic tensors are rejected rather than approximated.
+    """
+    t = np.asarray(tensor)
+    if t.ndim != 3 or t.shape[:2] != (3, 3):
+        raise ValueError(f"{name} must have shape (3, 3, T)")
+    d = t[0, 0]
+    off = np.ones((3, 3), dtype=bool)
+    np.fill_diagonal(off, False)
+    if np.any(t[off] != 0) or np.any(t[1, 1] != d) or np.any(t[2, 2] != d):
+        raise NotImplementedError(f"{name}: anisotropic material tensors are not supported by the HIP "
+                                  "element kernel")
+    return np.ascontiguousarray(d, dtype=np.complex128)
+
+
+class MeshOnDevice:
+    """A mesh uploaded to one context with its CSR pattern built."""
+
+    def __init__(self, ctx: Context, nodes: np.ndarray, tets: np.ndarray, tet_to_field: np.ndarray,
+                 n_field: int):
+        self.ctx = ctx
+        ctx.mesh_set(nodes, tets, tet_to_field, n_field)
+        self.nnz = ctx.pattern_build()
+        self.n_field = int(n_field)
+        self.n_tets = tets.shape[1]
+        self._pattern = None
+
+    @property
+    def pattern(self) -> tuple[np.ndarray, np.ndarray]:
+        if self._pattern is None:
+            self._pattern = self.ctx.pattern_get()
+        return self._pattern
+
+    def assemble(self, eps_r: np.ndarray, inv_mu_r: np.ndarray) -> None:
+        self.ctx.material_set(eps_r, inv_mu_r)
+        self.ctx.assemble()
+
+    def csr(self) -> tuple[csr_matrix, csr_matrix]:
+        indptr, indices = self.pattern
+        Ev, Bv = self.ctx.values_get()
+        N = self.n_field
+        E = csr_matrix((Ev, indices, indptr), shape=(N, N))
+        B = csr_matrix((Bv, indices, indptr), shape=(N, N))
+        E.has_sorted_indices = True
+        B.has_sorted_indices = True
+        return E, B
+
+
+class HipAssembler:
+    """Assembler plug-in: volume matrices on the GPU."""
+
+    def __init__(self, settings=None, device: int = 0):
+        self.settings = settings
+        self.cached_matrices = None
+        self.device = device
+        self._ctx: Context | None = None
+        self._mesh_key = None
+        self._dev_mesh: MeshOnDevice | None = None
+
+    def _context(self) -> Context:
+        if self._ctx is None:
+            self._ctx = Context(self.device)
+        return self._ctx
+
+    def device_mesh(self, field) -> MeshOnDevice:
+        mesh = field.mesh
+        key = (id(mesh), mesh.tets.shape, field.tet_to_field.shape)
+        if self._dev_mesh is None or self._mesh_key != key:
+            self._dev_mesh = MeshOnDevice(self._context(), mesh.nodes, mesh.tets, field.tet_to_field,
+                                          field.n_field)
+            self._mesh_key = key
+        return self._dev_mesh
+
+    def tet_mass_stiffness_matrices(self, field, er: np.ndarray, ur: np.ndarray):
+        """ASM/curlcurl.py:145-173: (E curl-curl, B mass) as scipy CSR."""
+        eps = isotropic_scalars(er, "er")
+        mu = isotropic_scalars(ur, "ur")
+        dm = self.device_mesh(field)
+        dm.assemble(eps, 1.0 / mu)  # matinv's diagonal path (E/mth/optimized.py:354-357)
+        return dm.csr()

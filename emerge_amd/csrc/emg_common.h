@@ -1,0 +1,123 @@
+// Internal definitions shared by the HIP translation units of libemerge_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/emerge_hip.h"
+
+namespace emg {
+
+// Owning device buffer (hipMalloc / hipFree), sized in elements of T.
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count == n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = count;
+        return hipSuccess;
+    }
+};
+
+// Work unit of the row-owner assembly: a run of consecutive CSR rows whose
+// incidences fit one workgroup and whose CSR slots fit the LDS accumulator.
+struct Chunk {
+    int32_t inc0;   // first incidence (index into inc arrays)
+    int32_t ninc;   // number of incidences (<= kAsmThreads)
+    int32_t slot0;  // first CSR slot of the chunk (= indptr[row0])
+    int32_t nslot;  // number of CSR slots (<= kAsmMaxSlots)
+    int32_t nrank;  // max incidences per row in the chunk
+    int32_t pad[3];
+};
+
+constexpr int kAsmThreads = 256;
+constexpr int kAsmMaxSlots = 2048;
+
+}  // namespace emg
+
+struct emg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    // mesh
+    int64_t n_nodes = 0, n_tets = 0, n_field = 0;
+    emg::DBuf<double> xyz;        // 3 x V SoA
+    emg::DBuf<int32_t> tets;      // 4 x T
+    emg::DBuf<int32_t> t2f;       // 20 x T
+
+    // pattern
+    bool have_pattern = false;
+    int64_t nnz = 0;
+    emg::DBuf<int32_t> indptr;    // N + 1
+    emg::DBuf<int32_t> indices;   // nnz
+    emg::DBuf<int32_t> cmap;      // 400 x T COO -> CSR slot
+    // row-owner assembly schedule
+    emg::DBuf<emg::Chunk> chunks;
+    int64_t n_chunks = 0, n_inc = 0, max_rank = 0;
+    emg::DBuf<int32_t> inc_tet;   // tet id per incidence (chunk-local order: by canonical row)
+    emg::DBuf<uint8_t> inc_u;     // canonical element row of the incidence
+    emg::DBuf<uint16_t> slot_beg; // per CSR slot: first chunk-relative contributor
+    emg::DBuf<uint16_t> contrib;  // 20 per incidence: (lane << 5) | canonical column, grouped by slot
+
+    // materials and values
+    bool have_material = false;
+    emg::DBuf<double> eps;        // 2 x T (re, im)
+    emg::DBuf<double> inu;        // 2 x T
+    emg::DBuf<double> Ev;         // 2 x nnz
+    emg::DBuf<double> Bv;         // 2 x nnz
+    bool have_values = false;
+};
+
+namespace emg {
+
+inline int fail(emg_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+}  // namespace emg
+
+#define EMG_HIP(ctx, call)                                                      \
+    do {                                                                        \
+        hipError_t _e = (call);                                                 \
+        if (_e != hipSuccess)                                                   \
+            return emg::fail((ctx), _e == hipErrorOutOfMemory ? EMG_ERR_NOMEM   \
+                                                               : EMG_ERR_HIP,  \
+                             "%s failed: %s (%s:%d)", #call,                    \
+                             hipGetErrorString(_e), __FILE__, __LINE__);        \
+    } while (0)
+
+#define EMG_CHECK_CTX(ctx)                                                      \
+    do {                                                                        \
+        if (!(ctx)) return EMG_ERR_ARG;                                         \
+        (void)hipSetDevice((ctx)->device);                                      \
+    } while (0)

@@ -1,0 +1,104 @@
+/*
+ * emerge_hip.h -- C-ABI of the MI355X (gfx950) FEM hot path for EMerge.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (0 = OK, < 0 = error; the message is read with emg_last_error).  No
+ * exception crosses this boundary.  Host buffers are caller-owned and only
+ * read/written during the call; device buffers belong to the context and are
+ * released by emg_destroy.  One HIP stream per context; a context is not
+ * re-entrant, distinct contexts may be used from distinct threads (the
+ * Python binding releases the GIL in every call).
+ *
+ * Reference paths below are relative to the EMerge repository,
+ *   E/   = src/emerge/_emerge/
+ *   ASM/ = src/emerge/_emerge/physics/microwave/assembly/
+ * The binding style follows the reference's ctypes PARDISO binding
+ * (E/solve_interfaces/pardiso_interface.py:188-221, 406-457).
+ */
+#ifndef EMERGE_HIP_H
+#define EMERGE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct emg_ctx emg_ctx;
+
+enum {
+    EMG_OK = 0,
+    EMG_ERR_ARG = -1,      /* invalid argument / inconsistent sizes        */
+    EMG_ERR_STATE = -2,    /* call out of order (e.g. assemble before mesh) */
+    EMG_ERR_HIP = -3,      /* HIP runtime error                            */
+    EMG_ERR_NOMEM = -4,    /* device allocation failed                     */
+    EMG_ERR_CONVERGE = -5  /* Krylov solve did not reach the tolerance     */
+};
+
+/* ---------------------------------------------------------------- context */
+
+/* Create a context on HIP device `device` (its own stream).
+ * Replaces the solver/assembler instance state the reference keeps per
+ * object (ASM/assembler.py:105-108 cached_matrices; E/solver.py:541-549). */
+int emg_create(int device, emg_ctx** out);
+int emg_destroy(emg_ctx* ctx);
+/* Copy the last error message of `ctx` (NUL-terminated, truncated to len). */
+int emg_last_error(const emg_ctx* ctx, char* buf, size_t len);
+/* Library version string "major.minor.patch". */
+const char* emg_version(void);
+
+/* Stream-ordered timing on the context stream (hipEvent pair). */
+int emg_timer_start(emg_ctx* ctx);
+int emg_timer_stop(emg_ctx* ctx, float* elapsed_ms);
+int emg_synchronize(emg_ctx* ctx);
+
+/* ------------------------------------------------------------------- mesh */
+
+/* Upload the mesh: xyz is SoA (3 x n_nodes, row-major: all x, then y, then z),
+ * tets (4 x n_tets) and tet_to_field (20 x n_tets) are row-major int32, i.e.
+ * exactly Mesh3D.nodes / Mesh3D.tets / Nedelec2.tet_to_field
+ * (E/mesh3d.py:343-360, E/elements/nedelec2.py:42-46) narrowed to int32.
+ * n_field = 2*nE + 2*nF (E/elements/nedelec2.py:35). */
+int emg_mesh_set(emg_ctx* ctx, int64_t n_nodes, const double* xyz,
+                 int64_t n_tets, const int32_t* tets,
+                 const int32_t* tet_to_field, int64_t n_field);
+
+/* ---------------------------------------------------------------- pattern */
+
+/* Build the global CSR sparsity pattern of the volume matrices and the
+ * COO->CSR scatter map on the device.  The pattern equals what
+ * scipy's coo_matrix((data,(rows,cols))).tocsr() yields for the COO layout of
+ * ASM/curlcurl.py:459-465 (duplicates merged, columns sorted, explicit zeros
+ * kept, int32 indices) -- the pattern of E and B of
+ * tet_mass_stiffness_matrices (ASM/curlcurl.py:145-173).  Returns nnz. */
+int emg_pattern_build(emg_ctx* ctx, int64_t* nnz);
+/* Copy indptr (n_field + 1) and indices (nnz) to host buffers. */
+int emg_pattern_get(emg_ctx* ctx, int32_t* indptr, int32_t* indices);
+/* Copy the COO->CSR scatter map (400 * n_tets): entry t*400 + 20*i + j holds
+ * the CSR slot of (tet_to_field[i,t], tet_to_field[j,t]). */
+int emg_scatter_map_get(emg_ctx* ctx, int32_t* map);
+/* Row-chunk statistics of the assembly schedule (for reports). */
+int emg_pattern_stats(emg_ctx* ctx, int64_t* n_chunks, int64_t* n_incidences,
+                      int64_t* max_rank);
+
+/* --------------------------------------------------------------- assembly */
+
+/* Upload per-tet isotropic material scalars, complex128 interleaved
+ * (re, im) of length 2*n_tets: eps_r (after the loss update of
+ * ASM/assembler.py:240) and inv_mu_r = 1/mu_r (E/mth/optimized.py:354-357). */
+int emg_material_set(emg_ctx* ctx, const double* eps_r, const double* inv_mu_r);
+
+/* Assemble E (curl-curl) and B (mass) CSR values on the device.
+ * Replaces tet_mass_stiffness_matrices / _matrix_builder /
+ * ned2_tet_stiff_mass (ASM/curlcurl.py:145-173, 419-466, 180-412) plus the
+ * COO->CSR duplicate sum.  Stream-ordered, asynchronous. */
+int emg_assemble(emg_ctx* ctx);
+/* Copy the assembled values (complex128 interleaved, 2*nnz doubles each). */
+int emg_values_get(emg_ctx* ctx, double* E_vals, double* B_vals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMERGE_HIP_H */

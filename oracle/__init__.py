@@ -1,0 +1,27 @@
+"""CPU oracle for the EMerge frequency-domain FEM hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import, call, link or run anything in
+this package, and only as the checker (or the timed CPU baseline) -- never as
+the product path.  The product (`emerge_amd`) never imports it.
+
+Every function restates a reference function from its source text (file:line
+cited in each docstring; paths are relative to the reference repository,
+``E/`` = ``src/emerge/_emerge/``, ``ASM/`` = ``E/physics/microwave/assembly/``).
+The reference itself cannot be imported here (numba, gmsh, loguru and emsutil
+are absent), so the restatement is pinned independently:
+
+* topology / DOF maps: brute-force set/dict restatement on tiny meshes
+  (`oracle.topology`) and the closed-form Kuhn counts;
+* element matrices: brute-force quadrature of the basis as defined by the
+  reference's interpolation routine (`oracle.quadrature`), symmetry, the
+  curl-kernel invariant (gradients of P2 scalars lie in the curl null space),
+  and the h / h^3 scale laws;
+* COO -> CSR: scipy 1.15.3 itself (the version the reference's lock file pins);
+* Robin / port surface terms: 2-D quadrature of the tangential basis;
+* direct solve: scipy SuperLU with the reference's options;
+* S-parameters: closed forms of an empty rectangular waveguide
+  (|S21| -> 1, arg S21 = -beta L, S11 -> 0, reciprocity).
+
+Pinning status per function is listed in each module header.
+"""

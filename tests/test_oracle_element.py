@@ -1,0 +1,132 @@
+"""Pins of the element-matrix oracle (oracle.curlcurl) that do not retype it.
+
+* brute-force quadrature of the basis as the reference's interpolation
+  routine defines it, plus the closed-form size of the reference's one
+  coefficient quirk in the face-face mass block;
+* symmetry, the 9-dimensional gradient null space of the curl-curl matrix,
+  the h / h^3 scale laws;
+* volume coefficients against exact rational Dirichlet integrals.
+"""
+from fractions import Fraction
+from math import factorial
+
+import numpy as np
+import pytest
+
+from emerge_amd.basis import Nedelec2
+from emerge_amd.mesh import Mesh3D, kuhn_box
+from oracle import curlcurl as cc
+from oracle import quadrature as qd
+
+
+def _mesh(cells=(2, 2, 2), size=(0.02, 0.015, 0.01), jitter=0.15, seed=3):
+    nodes, tets = kuhn_box(*cells, size=size, jitter=jitter, seed=seed)
+    m = Mesh3D(nodes, tets)
+    return m, Nedelec2(m)
+
+
+def _oracle(m, f, eps, mu):
+    lem, ltm = cc.local_maps(m.tets, m.edges, m.tris, f.tet_to_field, m.n_edges)
+    D, F = cc.ned2_tet_stiff_mass(m.nodes[:, m.tets], m.edge_lengths[m.tet_to_edge], lem, ltm,
+                                  cc.matinv(cc.isotropic(mu)), cc.isotropic(eps))
+    return D, F, lem, ltm
+
+
+def _quirk_correction(m, t, ltm, eps_t):
+    """Closed-form difference between the reference's face-face mass entries
+    and the exact integral: the reference uses the volume coefficient of
+    lam_B lam_C lam_D lam_F where the integral has lam_B lam_C lam_D lam_E
+    (faces (A,B,E) and (C,D,F) with sorted vertices)."""
+    X = m.nodes[:, m.tets[:, t]]
+    b, c, d, V = cc.tet_coefficients_bcd(X[0][:, None], X[1][:, None], X[2][:, None])
+    GL = np.stack([b[:, 0], c[:, 0], d[:, 0]], axis=1)
+    V = V[0]
+    dist = lambda p, q: np.linalg.norm(X[:, p] - X[:, q])
+    corr = np.zeros((20, 20), dtype=complex)
+    for i in range(4):
+        a_, b_, e_ = ltm[:, i, t]
+        for j in range(4):
+            c_, d_, f_ = ltm[:, j, t]
+            dv = (cc.VOLUME_COEFF_CACHE_BASE[b_ + 1, c_ + 1, d_ + 1, f_ + 1]
+                  - cc.VOLUME_COEFF_CACHE_BASE[b_ + 1, c_ + 1, d_ + 1, e_ + 1]) * 6 * V / (6 * V) ** 2
+            g_af = eps_t * GL[a_] @ GL[f_]
+            corr[6 + i, 6 + j] = -dist(a_, e_) * dist(c_, f_) * dv * g_af
+            corr[16 + i, 6 + j] = dist(a_, b_) * dist(c_, f_) * dv * g_af
+    return corr
+
+
+def test_quadrature_pin_with_quirk():
+    m, f = _mesh()
+    T = m.n_tets
+    rng = np.random.default_rng(0)
+    eps = rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T))
+    mu = np.full(T, 1.3 - 0.2j)
+    D, F, lem, ltm = _oracle(m, f, eps, mu)
+    n_quirk = 0
+    for t in range(0, T, 5):
+        K, M = qd.element_matrices_quadrature(
+            m.nodes[:, m.tets[:, t]], m.tets[:, t], m.edges[:, f.tet_to_field[:6, t]],
+            m.tris[:, f.tet_to_field[6:10, t] - m.n_edges], np.eye(3) / mu[t], np.eye(3) * eps[t])
+        np.testing.assert_allclose(D[:, :, t], K, rtol=0, atol=1e-10 * np.abs(K).max())
+        corr = _quirk_correction(m, t, ltm, eps[t])
+        n_quirk += np.count_nonzero(np.abs(corr) > 1e-12 * np.abs(M).max())
+        np.testing.assert_allclose(F[:, :, t], M + corr, rtol=0, atol=1e-10 * np.abs(M).max())
+    assert n_quirk > 0  # the quirk is exercised
+
+
+def test_quadrature_detects_planted_error():
+    """A transposed operand in the oracle's mass block must fail the pin."""
+    m, f = _mesh(cells=(1, 1, 1), jitter=0.0)
+    eps = np.ones(m.n_tets, dtype=complex)
+    D, F, lem, ltm = _oracle(m, f, eps, eps)
+    t = 2
+    K, M = qd.element_matrices_quadrature(
+        m.nodes[:, m.tets[:, t]], m.tets[:, t], m.edges[:, f.tet_to_field[:6, t]],
+        m.tris[:, f.tet_to_field[6:10, t] - m.n_edges], np.eye(3), np.eye(3))
+    planted = F[:, :, t].copy()
+    planted[:6, 6:10] = planted[:6, 6:10][::-1]
+    corr = _quirk_correction(m, t, ltm, 1.0)
+    assert np.abs(planted - (M + corr)).max() > 1e-6 * np.abs(M).max()
+
+
+def test_curl_null_space_and_symmetry():
+    m, f = _mesh(cells=(2, 1, 1))
+    T = m.n_tets
+    D, F, _, _ = _oracle(m, f, np.full(T, 2.0 + 0j), np.ones(T, dtype=complex))
+    for t in range(T):
+        K = D[:, :, t].real
+        assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+        sv = np.linalg.svd(K, compute_uv=False)
+        assert np.sum(sv < 1e-10 * sv[0]) == 9  # grad(P2) has dimension 10 - 1
+        assert np.sum(sv > 1e-8 * sv[0]) == 11
+
+
+def test_scale_laws():
+    m1, f1 = _mesh(cells=(1, 1, 1), size=(1.0, 1.0, 1.0), jitter=0.0)
+    m2, f2 = _mesh(cells=(1, 1, 1), size=(3.0, 3.0, 3.0), jitter=0.0)
+    one = np.ones(m1.n_tets, dtype=complex)
+    D1, F1, _, _ = _oracle(m1, f1, one, one)
+    D2, F2, _, _ = _oracle(m2, f2, one, one)
+    np.testing.assert_allclose(D2, 3.0 * D1, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(F2, 27.0 * F1, rtol=1e-12, atol=1e-14)
+
+
+def test_volume_coeff_exact():
+    for a in range(5):
+        for b in range(5):
+            for c in range(5):
+                for d in range(5):
+                    k = [0] * 5
+                    for x in (a, b, c, d):
+                        k[x] += 1
+                    k = k[1:]
+                    exact = Fraction(np.prod([factorial(x) for x in k]), factorial(sum(k) + 3))
+                    assert cc.VOLUME_COEFF_CACHE_BASE[a, b, c, d] == pytest.approx(float(exact), rel=1e-15)
+
+
+def test_matinv_general_path():
+    rng = np.random.default_rng(1)
+    M = rng.normal(size=(3, 3, 4)) + 1j * rng.normal(size=(3, 3, 4))
+    inv = cc.matinv(M)
+    for t in range(4):
+        np.testing.assert_allclose(inv[:, :, t] @ M[:, :, t], np.eye(3), atol=1e-12)
