@@ -45,10 +45,10 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
         const int64_t inc = int64_t(ch.inc0) + tid;
         const int t = inc_tet[inc];
         u = inc_u[inc];
-        int w[4];
+        int w[4], rk[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) w[k] = tets[k * T + t];
-        ned2::sort4(w);
+        ned2::ranks4(w, rk);
         double X[4][3];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
             X[k][1] = xyz[V + w[k]];
             X[k][2] = xyz[2 * V + w[k]];
         }
-        ned2::tet_geometry(X, G);
+        ned2::tet_geometry(X, rk, G);
         double su = 0.0;
 #pragma unroll
         for (int k = 0; k < 20; ++k) su = (k == u) ? G.s[k] : su;

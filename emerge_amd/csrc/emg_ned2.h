@@ -204,35 +204,57 @@ struct TetGeom {
     double inv6v;
 };
 
-__device__ __forceinline__ void sort4(int* w) {
-    // 5-comparator network
-#define EMG_CS(a, b) { int lo = min(w[a], w[b]); int hi = max(w[a], w[b]); w[a] = lo; w[b] = hi; }
-    EMG_CS(0, 1) EMG_CS(2, 3) EMG_CS(0, 2) EMG_CS(1, 3) EMG_CS(1, 2)
-#undef EMG_CS
+// rank of each of 4 distinct global ids
+__device__ __forceinline__ void ranks4(const int* w, int* rk) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rk[k] = (w[0] < w[k]) + (w[1] < w[k]) + (w[2] < w[k]) + (w[3] < w[k]);
 }
 
-// X[k] = coordinates of the vertex of rank k (global-id order).
-__device__ __forceinline__ void tet_geometry(const double (&X)[4][3], TetGeom& G) {
-    const double x1 = X[0][0], x2 = X[1][0], x3 = X[2][0], x4 = X[3][0];
-    const double y1 = X[0][1], y2 = X[1][1], y3 = X[2][1], y4 = X[3][1];
-    const double z1 = X[0][2], z2 = X[1][2], z3 = X[2][2], z4 = X[3][2];
-    // cofactors as in ASM/curlcurl.py:125-136
-    double gl[4][3];
-    gl[0][0] = -y2*z3 + y2*z4 + y3*z2 - y3*z4 - y4*z2 + y4*z3;
-    gl[1][0] = y1*z3 - y1*z4 - y3*z1 + y3*z4 + y4*z1 - y4*z3;
-    gl[2][0] = -y1*z2 + y1*z4 + y2*z1 - y2*z4 - y4*z1 + y4*z2;
-    gl[3][0] = y1*z2 - y1*z3 - y2*z1 + y2*z3 + y3*z1 - y3*z2;
-    gl[0][1] = x2*z3 - x2*z4 - x3*z2 + x3*z4 + x4*z2 - x4*z3;
-    gl[1][1] = -x1*z3 + x1*z4 + x3*z1 - x3*z4 - x4*z1 + x4*z3;
-    gl[2][1] = x1*z2 - x1*z4 - x2*z1 + x2*z4 + x4*z1 - x4*z2;
-    gl[3][1] = -x1*z2 + x1*z3 + x2*z1 - x2*z3 - x3*z1 + x3*z2;
-    gl[0][2] = -x2*y3 + x2*y4 + x3*y2 - x3*y4 - x4*y2 + x4*y3;
-    gl[1][2] = x1*y3 - x1*y4 - x3*y1 + x3*y4 + x4*y1 - x4*y3;
-    gl[2][2] = -x1*y2 + x1*y4 + x2*y1 - x2*y4 - x4*y1 + x4*y2;
-    gl[3][2] = x1*y2 - x1*y3 - x2*y1 + x2*y3 + x3*y1 - x3*y2;
-    // 6V = |sum_i x_i b_i| (linear precision of the barycentric basis)
-    const double six_v = fabs(x1 * gl[0][0] + x2 * gl[1][0] + x3 * gl[2][0] + x4 * gl[3][0]);
-    G.inv6v = 1.0 / six_v;
+// XL[k] = coordinates of tet-local vertex k (the mesh's own vertex order),
+// rk[k] = rank of its global id.  Cofactors and volume use the reference's
+// polynomials in local order (ASM/curlcurl.py:111-136) so that they round
+// exactly as the reference does, then move to the canonical frame.
+__device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int (&rk)[4], TetGeom& G) {
+    // The 24-term volume and the cofactors cancel heavily for small tets far
+    // from the origin; evaluate them with plain IEEE multiply/add (no FMA
+    // contraction) so they round exactly like the reference.
+#pragma clang fp contract(off)
+    const double x1 = XL[0][0], x2 = XL[1][0], x3 = XL[2][0], x4 = XL[3][0];
+    const double y1 = XL[0][1], y2 = XL[1][1], y3 = XL[2][1], y4 = XL[3][1];
+    const double z1 = XL[0][2], z2 = XL[1][2], z3 = XL[2][2], z4 = XL[3][2];
+    double gll[4][3];
+    gll[0][0] = -y2*z3 + y2*z4 + y3*z2 - y3*z4 - y4*z2 + y4*z3;
+    gll[1][0] = y1*z3 - y1*z4 - y3*z1 + y3*z4 + y4*z1 - y4*z3;
+    gll[2][0] = -y1*z2 + y1*z4 + y2*z1 - y2*z4 - y4*z1 + y4*z2;
+    gll[3][0] = y1*z2 - y1*z3 - y2*z1 + y2*z3 + y3*z1 - y3*z2;
+    gll[0][1] = x2*z3 - x2*z4 - x3*z2 + x3*z4 + x4*z2 - x4*z3;
+    gll[1][1] = -x1*z3 + x1*z4 + x3*z1 - x3*z4 - x4*z1 + x4*z3;
+    gll[2][1] = x1*z2 - x1*z4 - x2*z1 + x2*z4 + x4*z1 - x4*z2;
+    gll[3][1] = -x1*z2 + x1*z3 + x2*z1 - x2*z3 - x3*z1 + x3*z2;
+    gll[0][2] = -x2*y3 + x2*y4 + x3*y2 - x3*y4 - x4*y2 + x4*y3;
+    gll[1][2] = x1*y3 - x1*y4 - x3*y1 + x3*y4 + x4*y1 - x4*y3;
+    gll[2][2] = -x1*y2 + x1*y4 + x2*y1 - x2*y4 - x4*y1 + x4*y2;
+    gll[3][2] = x1*y2 - x1*y3 - x2*y1 + x2*y3 + x3*y1 - x3*y2;
+    const double vol = fabs(-x1*y2*z3/6 + x1*y2*z4/6 + x1*y3*z2/6 - x1*y3*z4/6 - x1*y4*z2/6 + x1*y4*z3/6
+                            + x2*y1*z3/6 - x2*y1*z4/6 - x2*y3*z1/6 + x2*y3*z4/6 + x2*y4*z1/6 - x2*y4*z3/6
+                            - x3*y1*z2/6 + x3*y1*z4/6 + x3*y2*z1/6 - x3*y2*z4/6 - x3*y4*z1/6 + x3*y4*z2/6
+                            + x4*y1*z2/6 - x4*y1*z3/6 - x4*y2*z1/6 + x4*y2*z3/6 + x4*y3*z1/6 - x4*y3*z2/6);
+    G.inv6v = 1.0 / (6.0 * vol);
+    // canonical frame: gl[c] / X[c] belong to the local vertex of rank c
+    double gl[4][3], X[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double gv = gll[0][d], xv = XL[0][d];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) {
+                gv = rk[k] == c ? gll[k][d] : gv;
+                xv = rk[k] == c ? XL[k][d] : xv;
+            }
+            gl[c][d] = gv;
+            X[c][d] = xv;
+        }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
