@@ -37,6 +37,18 @@ SIGNATURES: dict[str, tuple] = {
     "emg_material_set": (_c_int, [_p, _p, _p]),
     "emg_assemble": (_c_int, [_p]),
     "emg_values_get": (_c_int, [_p, _p, _p]),
+    "emg_system_reduce": (_c_int, [_p, _i64, _p]),
+    "emg_system_pattern_get": (_c_int, [_p, _p, _p, _pi64]),
+    "emg_bc_add": (_c_int, [_p, _i64, _p, _p, ctypes.POINTER(ctypes.c_int32), _pi64]),
+    "emg_tri_terms_get": (_c_int, [_p, _p, _p]),
+    "emg_surface_set": (_c_int, [_p, _i64, _p, _p, _p, _p, _i64, _p]),
+    "emg_surface_get": (_c_int, [_p, _p]),
+    "emg_forcing_set": (_c_int, [_p, ctypes.c_int32, _i64, _p, _p, _i64, _p]),
+    "emg_forcing_get": (_c_int, [_p, _p]),
+    "emg_sweep_solve": (_c_int, [_p, _i64, _p, _p, _p, _c_int, _c_int, ctypes.c_double, _c_int, _c_int,
+                                 _p, _p, _p]),
+    "emg_csr_set": (_c_int, [_p, _i64, _p, _p, _p]),
+    "emg_csr_solve": (_c_int, [_p, ctypes.c_int32, _p, _c_int, ctypes.c_double, _c_int, _c_int, _p, _p, _p]),
 }
 
 _lib = None

@@ -17,6 +17,8 @@ SOURCES = [
     "emg_context.hip",
     "emg_pattern.hip",
     "emg_assemble.hip",
+    "emg_surface.hip",
+    "emg_solve.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")

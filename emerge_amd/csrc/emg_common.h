@@ -53,7 +53,7 @@ struct Chunk {
 };
 
 constexpr int kAsmThreads = 256;
-constexpr int kAsmMaxSlots = 2048;
+constexpr int kAsmMaxSlots = 4096;
 
 }  // namespace emg
 
@@ -90,6 +90,33 @@ struct emg_ctx {
     emg::DBuf<double> Ev;         // 2 x nnz
     emg::DBuf<double> Bv;         // 2 x nnz
     bool have_values = false;
+
+    // PEC-reduced system (rows/cols of eliminated DOFs removed)
+    bool have_system = false;
+    int64_t n_red = 0, nnz_red = 0;
+    emg::DBuf<int32_t> keep;      // N: reduced index or -1
+    emg::DBuf<int32_t> red_ptr, red_idx, red_map, red_diag;
+    emg::DBuf<double> E64, B64;   // 2 x nnz_red
+    emg::DBuf<float> E32, B32;    // 2 x nnz_red
+
+    // boundary triangles (all BCs concatenated) and reduced surface terms
+    int64_t n_tri_total = 0;
+    int32_t n_bc = 0;
+    emg::DBuf<double> tri_mat;    // 64 per triangle (no gamma)
+    emg::DBuf<double2> tri_force; // 8 per triangle (zero when not excited)
+    bool have_surface = false;
+    int64_t n_sentries = 0;
+    emg::DBuf<int32_t> s_ptr, s_col, s_bc;
+    emg::DBuf<double> s_val64;
+    emg::DBuf<float> s_val32;
+    int32_t n_forcing = 0;
+    emg::DBuf<double> forcing;    // 2 x n_forcing x n_red
+
+    // generic CSR system handed in by the Solver plug-in
+    int64_t g_n = 0, g_nnz = 0;
+    emg::DBuf<int32_t> g_ptr, g_idx, g_diag;
+    emg::DBuf<double> g_v64;
+    emg::DBuf<float> g_v32;
 };
 
 namespace emg {

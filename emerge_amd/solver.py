@@ -1,0 +1,118 @@
+"""Drop-in linear solver plug-in backed by the HIP Krylov kernels.
+
+``SolverHIP`` has the shape of the reference ``Solver`` base class
+(E/solver.py:333-371): class attributes ``real_only``, ``req_sorter`` and
+``released_gil``; ``initialize / duplicate / set_options / reset``; and
+``solve(A, b, precon, reuse_factorization, id) -> (x, SolveReport)``.  It is
+registered like any other solver, ``model.set_solver(SolverHIP())``
+(E/simmodel.py:726-737, E/solver.py:1079-1090), and sits next to SuperLU,
+UMFPACK, PARDISO and cuDSS (E/solver.py:535-773).  ctypes calls release the
+GIL, so the solver is legal in the multi-threaded sweep mode
+(E/solver.py:1031-1033).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Callable
+
+import numpy as np
+from scipy.sparse import csr_matrix
+
+from ._lib import Context, EmgError, ptr
+
+
+class SimulationError(Exception):
+    """Raised on solver failure (as E/solver.py:295-296)."""
+
+
+@dataclass
+class SolveReport:
+    """Per-solve record with the fields of E/solver.py:92-113."""
+    simtime: float = -1.0
+    jobid: int = -1
+    ndof: int = -1
+    nnz: int = -1
+    ndof_solve: int = -1
+    nnz_solve: int = -1
+    exit_code: int = 0
+    solver: str = "None"
+    sorter: str = "None"
+    precon: str = "None"
+    aux: dict = field(default_factory=dict)
+    worker_name: str = "Unknown Worker"
+
+    def add(self, **kwargs):
+        for k, v in kwargs.items():
+            self.aux[k] = str(v)
+
+    @property
+    def mdof(self) -> float:
+        return (self.ndof ** 2) / ((self.simtime + 1e-6) * 1e6)
+
+
+class SolverHIP:
+    """GMRES on the GPU (FP64, or complex64 inside FP64 iterative refinement)."""
+
+    real_only: bool = False
+    req_sorter: bool = False
+    released_gil: bool = True
+
+    def __init__(self, pre: str = "", precision: str = "fp64", tol: float = 1e-10, restart: int = 150,
+                 max_cycles: int = 400, device: int = 0):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.pre = pre
+        self.own_preconditioner = True
+        self.initialized = False
+        self.precision = precision
+        self.tol = tol
+        self.restart = restart
+        self.max_cycles = max_cycles
+        self.device = device
+        self._ctx: Context | None = None
+        self._n = -1
+
+    def __str__(self) -> str:
+        return "SolverHIP"
+
+    def initialize(self) -> None:
+        if self.initialized:
+            return
+        self._ctx = Context(self.device)
+        self.initialized = True
+
+    def duplicate(self) -> "SolverHIP":
+        return self.__class__(self.pre, self.precision, self.tol, self.restart, self.max_cycles, self.device)
+
+    def set_options(self, pivoting_threshold: float | None = None) -> None:
+        """Pivoting thresholds do not apply to a Krylov solver."""
+
+    def reset(self) -> None:
+        self._n = -1
+
+    def solve(self, A, b: np.ndarray, precon=None, reuse_factorization: bool = False, id: int = -1):
+        self.initialize()
+        t0 = time.time()
+        Ac = csr_matrix(A)  # SolveRoutine hands a CSC slice (E/solver.py:1266-1268)
+        Ac.sort_indices()
+        n = Ac.shape[0]
+        if not reuse_factorization or self._n != n:
+            ip = np.ascontiguousarray(Ac.indptr, dtype=np.int32)
+            ix = np.ascontiguousarray(Ac.indices, dtype=np.int32)
+            v = np.ascontiguousarray(Ac.data, dtype=np.complex128)
+            self._ctx.call("emg_csr_set", n, ptr(ip), ptr(ix), ptr(v))
+            self._n = n
+        bb = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(-1))
+        x = np.empty(n, dtype=np.complex128)
+        rel = np.empty(1)
+        it = np.empty(1, dtype=np.int64)
+        prec = 1 if self.precision == "mixed" else 0
+        try:
+            self._ctx.call("emg_csr_solve", 1, ptr(bb), prec, float(self.tol), int(self.restart),
+                           int(self.max_cycles), ptr(x), ptr(rel), ptr(it))
+        except EmgError as e:
+            raise SimulationError(str(e)) from e
+        aux = {"precision": self.precision, "iterations": str(int(it[0])), "relres": f"{rel[0]:.3e}",
+               "tolerance": str(self.tol)}
+        return x, SolveReport(solver=str(self), exit_code=0, aux=aux, simtime=time.time() - t0)

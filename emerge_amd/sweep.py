@@ -1,0 +1,228 @@
+"""Frequency sweep on the GPU: PEC elimination, Robin/port terms, batched solve.
+
+Host side of the path that replaces, per frequency, ``assemble_freq_matrix``
+(ASM/assembler.py:189-429) and ``SolveRoutine.solve`` (E/solver.py:1241-1326)
+inside ``Microwave3D.run_sweep`` (MW/microwave_3d.py:773-794).  Because E, B
+and every surface matrix share one pattern and only scalars depend on k0
+(K = E - k0^2 B + sum_bc gamma_bc(k0) S_bc, ASM/assembler.py:255, 349-350),
+the volume matrices are assembled once and all frequency points (times all
+excited ports, MW/simjob.py:82-108) are solved in batches on the device.
+
+Host work here is integer bookkeeping done once per mesh (which DOFs a PEC
+face removes, which triangle entries land in which reduced row); all
+floating-point work (element and surface integrals, their sums, the solves)
+runs in libemerge_hip.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import bc as bcs
+from ._lib import Context, ptr
+from .assembler import MeshOnDevice
+
+C0 = 299792458
+
+# gaus_quad_tri(4) (E/mth/optimized.py:31-32, 87-118): weights and
+# barycentric coordinates in generation order
+_DUN_W = np.array([0.223381589678011] * 3 + [0.109951743655322] * 3)
+_DUN_A = (0.108103018168070, 0.445948490915965)
+_DUN_B = (0.816847572980459, 0.091576213509771)
+_DUN_L = np.array([[_DUN_A[0], _DUN_A[1], _DUN_A[1]], [_DUN_A[1], _DUN_A[1], _DUN_A[0]],
+                   [_DUN_A[1], _DUN_A[0], _DUN_A[1]],
+                   [_DUN_B[0], _DUN_B[1], _DUN_B[1]], [_DUN_B[1], _DUN_B[1], _DUN_B[0]],
+                   [_DUN_B[1], _DUN_B[0], _DUN_B[1]]])
+
+
+def dunavant_points(nodes: np.ndarray, tris: np.ndarray, tri_ids: np.ndarray) -> np.ndarray:
+    """Quadrature points (n_tri, 6, 3) of gaus_quad_tri(4) on the triangles."""
+    v = nodes[:, tris[:, tri_ids]]  # (3 coords, 3 verts, n)
+    return np.einsum("qk,ckn->nqc", _DUN_L, v)
+
+
+@dataclass
+class SweepResult:
+    freqs: np.ndarray
+    ports: list
+    fields: dict            # (freq index, port number) -> full solution vector
+    relres: np.ndarray      # (nfreq, nports)
+    iters: np.ndarray
+    precision: str
+    timing: dict = field(default_factory=dict)
+
+
+class FrequencySweep:
+    """Device-resident E, B, surface terms and forcing of one mesh."""
+
+    def __init__(self, mesh, basis, eps_r: np.ndarray, inv_mu_r: np.ndarray, boundary_conditions: list,
+                 device: int = 0, ctx: Context | None = None):
+        self.mesh = mesh
+        self.basis = basis
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.dm = MeshOnDevice(self.ctx, mesh.nodes, mesh.tets, basis.tet_to_field, basis.n_field)
+        self.dm.assemble(eps_r, inv_mu_r)
+        self.bcs = list(boundary_conditions)
+        self.pec = [b for b in self.bcs if isinstance(b, bcs.PEC)]
+        self.robin = [b for b in self.bcs if isinstance(b, bcs.RobinBC)]
+        self.ports = sorted([b for b in self.robin if b.is_port], key=lambda p: p.port_number)
+        self._setup()
+
+    # ------------------------------------------------------------------ setup
+    def pec_dofs(self) -> np.ndarray:
+        """DOFs removed by PEC faces: both DOFs of every edge of a PEC triangle
+        and all 8 DOFs of the triangle (ASM/assembler.py:290-307)."""
+        f = self.basis
+        ids = [np.zeros(0, dtype=np.int64)]
+        for p in self.pec:
+            t = p.tri_ids
+            ids.append(f.edge_to_field[:, self.mesh.tri_to_edge[:, t].reshape(-1)].reshape(-1))
+            ids.append(f.tri_to_field[:, t].reshape(-1))
+        return np.unique(np.concatenate(ids))
+
+    def _setup(self) -> None:
+        f, m = self.basis, self.mesh
+        N = f.n_field
+        keep_mask = np.ones(N, dtype=bool)
+        keep_mask[self.pec_dofs()] = False
+        self.solve_ids = np.nonzero(keep_mask)[0]
+        red = np.full(N, -1, dtype=np.int64)
+        red[self.solve_ids] = np.arange(self.solve_ids.shape[0])
+        self.red = red
+        ids32 = np.ascontiguousarray(self.solve_ids, dtype=np.int32)
+        self.ctx.call("emg_system_reduce", ids32.shape[0], ptr(ids32))
+        self.n_red = ids32.shape[0]
+
+        # boundary triangles of every Robin BC on the device
+        tri_off = []
+        self._port_index = {}
+        for ib, b in enumerate(self.robin):
+            tv = np.ascontiguousarray(m.tris[:, b.tri_ids], dtype=np.int32)
+            u = None
+            if b.excited:
+                pts = dunavant_points(m.nodes, m.tris, b.tri_ids)  # (n,6,3)
+                shp = b.uinc_shape(pts[..., 0].reshape(-1), pts[..., 1].reshape(-1), pts[..., 2].reshape(-1))
+                u = np.ascontiguousarray(np.asarray(shp, dtype=np.complex128).reshape(3, -1).T)  # (n*6, 3)
+            bid = ctypes.c_int32()
+            off = ctypes.c_int64()
+            self.ctx.call("emg_bc_add", tv.shape[1], ptr(tv), ptr(u) if u is not None else None,
+                          ctypes.byref(bid), ctypes.byref(off))
+            tri_off.append(off.value)
+        self.tri_off = tri_off
+
+        # surface entries (row, col, bc) over reduced rows, contributions in tri order
+        rows_l, cols_l, bcs_l, con_l = [], [], [], []
+        for ib, b in enumerate(self.robin):
+            t = b.tri_ids
+            dof = f.tri_to_field[:, t]  # (8, n)
+            rr = red[dof]
+            n = t.shape[0]
+            g = tri_off[ib] + np.arange(n)
+            r = np.repeat(rr.T, 8, axis=1).reshape(-1)        # entry (tri, a, b) row dof[a]
+            c = np.tile(rr.T, (1, 8)).reshape(-1)             # col dof[b]
+            con = (g[:, None] * 64 + np.arange(64)[None, :]).reshape(-1)
+            ok = (r >= 0) & (c >= 0)
+            rows_l.append(r[ok]); cols_l.append(c[ok]); con_l.append(con[ok])
+            bcs_l.append(np.full(ok.sum(), ib))
+        if rows_l:
+            r = np.concatenate(rows_l); c = np.concatenate(cols_l)
+            bb = np.concatenate(bcs_l); con = np.concatenate(con_l)
+        else:
+            r = c = bb = con = np.zeros(0, dtype=np.int64)
+        order = np.lexsort((con, bb, c, r))
+        r, c, bb, con = r[order], c[order], bb[order], con[order]
+        key = (r * (self.n_red + 1) + c) * (len(self.robin) + 1) + bb
+        head = np.ones(key.shape[0], dtype=bool)
+        head[1:] = key[1:] != key[:-1]
+        starts = np.nonzero(head)[0]
+        seg = np.append(starts, key.shape[0]).astype(np.int32)
+        er, ec, eb = r[starts], c[starts], bb[starts]
+        s_ptr = np.zeros(self.n_red + 1, dtype=np.int32)
+        np.add.at(s_ptr, er + 1, 1)
+        s_ptr = np.cumsum(s_ptr).astype(np.int32)
+        a_col = np.ascontiguousarray(ec, dtype=np.int32)
+        a_bc = np.ascontiguousarray(eb, dtype=np.int32)
+        a_con = np.ascontiguousarray(con, dtype=np.int32)
+        self.ctx.call("emg_surface_set", a_col.shape[0], ptr(s_ptr), ptr(a_col), ptr(a_bc), ptr(seg),
+                      a_con.shape[0], ptr(a_con))
+        self.n_sentries = a_col.shape[0]
+        self.s_ptr, self.s_col, self.s_bc = s_ptr, a_col, a_bc
+
+        # forcing vectors: one per port, rows p*n_red + r
+        frows, fcon = [], []
+        for ip, p in enumerate(self.ports):
+            ib = self.robin.index(p)
+            t = p.tri_ids
+            rr = red[f.tri_to_field[:, t]].T  # (n, 8)
+            g = tri_off[ib] + np.arange(t.shape[0])
+            con = (g[:, None] * 8 + np.arange(8)[None, :]).reshape(-1)
+            rr = rr.reshape(-1)
+            ok = rr >= 0
+            frows.append(ip * self.n_red + rr[ok]); fcon.append(con[ok])
+        if self.ports:
+            fr = np.concatenate(frows); fc = np.concatenate(fcon)
+            order = np.lexsort((fc, fr))
+            fr, fc = fr[order], fc[order]
+            head = np.ones(fr.shape[0], dtype=bool)
+            head[1:] = fr[1:] != fr[:-1]
+            st = np.nonzero(head)[0]
+            fseg = np.append(st, fr.shape[0]).astype(np.int32)
+            frow = np.ascontiguousarray(fr[st], dtype=np.int32)
+            fcon_a = np.ascontiguousarray(fc, dtype=np.int32)
+            self.ctx.call("emg_forcing_set", len(self.ports), frow.shape[0], ptr(frow), ptr(fseg),
+                          fcon_a.shape[0], ptr(fcon_a))
+
+    # -------------------------------------------------------------- accessors
+    def reduced_pattern(self):
+        nnz = ctypes.c_int64()
+        self.ctx.call("emg_system_pattern_get", None, None, ctypes.byref(nnz))
+        ip = np.empty(self.n_red + 1, dtype=np.int32)
+        ix = np.empty(nnz.value, dtype=np.int32)
+        self.ctx.call("emg_system_pattern_get", ptr(ip), ptr(ix), ctypes.byref(nnz))
+        return ip, ix
+
+    def forcing(self) -> np.ndarray:
+        out = np.empty((len(self.ports), self.n_red), dtype=np.complex128)
+        self.ctx.call("emg_forcing_get", ptr(out))
+        return out
+
+    # ------------------------------------------------------------------ solve
+    def batch_coefficients(self, freqs: np.ndarray):
+        """k2, gamma [batch][bc] and rhs coefficients [batch][port] for the
+        (frequency, active port) pairs in frequency-major order."""
+        freqs = np.asarray(freqs, dtype=np.float64)
+        nb = len(self.robin)
+        npt = len(self.ports)
+        k0 = 2 * np.pi * freqs / C0
+        k2 = np.repeat(k0 ** 2, npt)
+        gam = np.zeros((freqs.shape[0] * npt, max(nb, 1)), dtype=np.complex128)
+        coef = np.zeros((freqs.shape[0] * npt, npt), dtype=np.complex128)
+        for i, k in enumerate(k0):
+            g = np.array([b.get_gamma(k) for b in self.robin], dtype=np.complex128)
+            for j, p in enumerate(self.ports):
+                row = i * npt + j
+                gam[row, :nb] = g
+                coef[row, j] = p.uinc_factor(k)
+        return np.ascontiguousarray(k2), np.ascontiguousarray(gam[:, :nb] if nb else gam), np.ascontiguousarray(coef)
+
+    def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock: int = 16, restart: int = 100,
+              max_cycles: int = 200) -> SweepResult:
+        freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+        k2, gam, coef = self.batch_coefficients(freqs)
+        nbatch = k2.shape[0]
+        x = np.empty((nbatch, self.n_red), dtype=np.complex128)
+        relres = np.empty(nbatch)
+        iters = np.empty(nbatch, dtype=np.int64)
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        self.ctx.call("emg_sweep_solve", nbatch, ptr(k2), ptr(gam) if gam.size else None, ptr(coef), prec,
+                      int(fblock), float(tol), int(restart), int(max_cycles), ptr(x), ptr(relres), ptr(iters))
+        npt = len(self.ports)
+        fields = {}
+        for i in range(freqs.shape[0]):
+            for j, p in enumerate(self.ports):
+                full = np.zeros(self.basis.n_field, dtype=np.complex128)
+                full[self.solve_ids] = x[i * npt + j]
+                fields[(i, p.port_number)] = full
+        return SweepResult(freqs, self.ports, fields, relres.reshape(-1, npt), iters.reshape(-1, npt), precision)

@@ -97,6 +97,72 @@ int emg_assemble(emg_ctx* ctx);
 /* Copy the assembled values (complex128 interleaved, 2*nnz doubles each). */
 int emg_values_get(emg_ctx* ctx, double* E_vals, double* B_vals);
 
+/* ------------------------------------------------------- reduced system */
+
+/* Keep only the DOFs in keep_ids (strictly increasing): the PEC elimination
+ * A[ix_(ids, ids)] of SolveRoutine.solve (E/solver.py:1266-1269) with
+ * solve_ids from ASM/assembler.py:274-307, 406-407.  Builds the reduced CSR
+ * pattern on the device and gathers the assembled E and B values into it
+ * (FP64 and FP32 copies). */
+int emg_system_reduce(emg_ctx* ctx, int64_t n_keep, const int32_t* keep_ids);
+/* Copy the reduced pattern (indptr n_keep + 1, indices nnz); either may be
+ * NULL.  nnz receives the reduced nonzero count. */
+int emg_system_pattern_get(emg_ctx* ctx, int32_t* indptr, int32_t* indices, int64_t* nnz);
+
+/* ------------------------------------------------------ surface / ports */
+
+/* Add the boundary triangles of one Robin/port/absorbing boundary condition:
+ * tri_vid (3 x n_tri row-major vertex ids), and for an excited port the
+ * incident field sampled at the 6 Dunavant points of gaus_quad_tri(4)
+ * (uinc, complex [n_tri][6][3], NULL otherwise).  The device computes per
+ * triangle the 8x8 matrix of ned2_tri_stiff / ned2_tri_stiff_force without
+ * its gamma factor and the forcing vector (ASM/robinbc.py:126-443).
+ * Triangles of all BCs are concatenated; *tri_offset is this BC's first. */
+int emg_bc_add(emg_ctx* ctx, int64_t n_tri, const int32_t* tri_vid, const double* uinc,
+               int32_t* bc_id, int64_t* tri_offset);
+/* Copy the per-triangle matrices (64 per triangle) and forcing vectors
+ * (8 complex per triangle); either may be NULL. */
+int emg_tri_terms_get(emg_ctx* ctx, double* mat, double* force);
+/* Surface entries on the reduced pattern, CSR over reduced rows: s_ptr
+ * (n_red + 1), per entry its column s_col and BC s_bc; entry k is the sum of
+ * the triangle matrix values contrib[seg[k] .. seg[k+1]) (index
+ * 64*triangle + 8*i + j), summed in the given order -- the Bempty /
+ * generate_csr / K + B_p steps of ASM/assembler.py:314-350 with gamma kept
+ * separate (K = E - k0^2 B + sum_bc gamma_bc S_bc). */
+int emg_surface_set(emg_ctx* ctx, int64_t n_entries, const int32_t* s_ptr, const int32_t* s_col,
+                    const int32_t* s_bc, const int32_t* seg, int64_t n_contrib, const int32_t* contrib);
+int emg_surface_get(emg_ctx* ctx, double* s_val);
+/* Dense reduced forcing vectors (port_vectors of ASM/assembler.py:267, 336):
+ * vector p, reduced row r is element rows[k] = p*n_red + r, the sum of the
+ * triangle forcing values contrib[seg[k] .. seg[k+1]) (index 8*triangle+i). */
+int emg_forcing_set(emg_ctx* ctx, int32_t n_vectors, int64_t n_rows, const int32_t* rows,
+                    const int32_t* seg, int64_t n_contrib, const int32_t* contrib);
+int emg_forcing_get(emg_ctx* ctx, double* out);
+
+/* ------------------------------------------------------------------ solve */
+
+/* Solve A_b x_b = rhs_b for nbatch systems on the reduced pattern:
+ *   A_b = E - k2[b] B + sum_bc gamma[b][bc] S_bc,  rhs_b = sum_p coef[b][p] f_p
+ * (gamma complex [nbatch][n_bc], coef complex [nbatch][n_vectors]).
+ * Replaces the per-frequency assemble + SolveRoutine.solve of run_sweep
+ * (MW/microwave_3d.py:773-794; E/solver.py:1241-1326).  precision 0 = FP64
+ * GMRES, 1 = complex64 GMRES inside FP64 iterative refinement; fblock = max
+ * systems per batch (1..32, matrix read once per batch).  x_out complex
+ * [nbatch][n_red]; relres_out = final FP64 ||b - A x|| / ||b||. */
+int emg_sweep_solve(emg_ctx* ctx, int64_t nbatch, const double* k2, const double* gamma,
+                    const double* rhs_coef, int precision, int fblock, double tol, int restart,
+                    int max_cycles, double* x_out, double* relres_out, int64_t* iters_out);
+
+/* Generic complex CSR system for the Solver plug-in (replaces
+ * Solver.solve(A, b, ...), E/solver.py:365-367, next to SuperLU / PARDISO /
+ * cuDSS, E/solver.py:535-773): vals complex128 interleaved. */
+int emg_csr_set(emg_ctx* ctx, int64_t n, const int32_t* indptr, const int32_t* indices,
+                const double* vals);
+/* nrhs right-hand sides b (complex [nrhs][n]) -> x (complex [nrhs][n]).
+ * Returns EMG_ERR_CONVERGE (x still written) if a system misses tol. */
+int emg_csr_solve(emg_ctx* ctx, int32_t nrhs, const double* b, int precision, double tol,
+                  int restart, int max_cycles, double* x, double* relres_out, int64_t* iters_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,131 @@
+"""GPU surface terms and Krylov solves against the oracle (SuperLU, reference options).
+
+Bar: surface/forcing values within 1e-12 relative; every solve reaches the
+FP64 relative residual ||b - A x|| / ||b|| <= 1e-8 (recomputed here with the
+oracle's matrix); S-parameters within |dS| <= 1e-6 of the SuperLU solution.
+"""
+import numpy as np
+import pytest
+
+from emerge_amd.solver import SolverHIP
+from emerge_amd.sweep import FrequencySweep
+from oracle import robin
+from tests import wg_case
+
+pytestmark = pytest.mark.gpu
+
+F0 = 10e9
+
+
+@pytest.fixture(scope="module")
+def sweep():
+    c = wg_case.case()
+    p = c["problem"]
+    return FrequencySweep(c["mesh"], c["basis"], p.eps_r, p.inv_mu_r, c["prod_bcs"])
+
+
+def test_triangle_terms_match_oracle(sweep):
+    c = wg_case.case()
+    m = c["mesh"]
+    n = sweep.ctx  # noqa
+    ntot = sum(b.tri_ids.shape[0] for b in sweep.robin)
+    mat = np.empty((ntot, 64))
+    force = np.empty((ntot, 8), dtype=np.complex128)
+    from emerge_amd._lib import ptr
+    sweep.ctx.call("emg_tri_terms_get", ptr(mat), ptr(force))
+    pts = robin.gaus_quad_tri(4)
+    port = sweep.ports[0]
+    off = sweep.tri_off[sweep.robin.index(port)]
+    from emerge_amd.sweep import dunavant_points
+    P = dunavant_points(m.nodes, m.tris, port.tri_ids)
+    for k, t in enumerate(port.tri_ids[:40]):
+        shp = port.uinc_shape(P[k, :, 0], P[k, :, 1], P[k, :, 2])
+        Bo, bo = robin.ned2_tri_stiff_force(m.nodes[:, m.tris[:, t]], 1.0, shp, pts)
+        np.testing.assert_allclose(mat[off + k], Bo.real.ravel(), rtol=0, atol=1e-12 * np.abs(Bo).max())
+        np.testing.assert_allclose(force[off + k], bo, rtol=0, atol=1e-12 * np.abs(bo).max())
+
+
+def test_reduced_operator_matches_oracle(sweep):
+    """E_red - k0^2 B_red + sum_bc gamma_bc S_bc built from the device pieces
+    equals the oracle's K[ids, ids] (K = E - k0^2 B + B_p)."""
+    from scipy.sparse import csr_matrix
+    from emerge_amd._lib import ptr
+    o = wg_case.oracle_solution(F0)
+    ids = o["ids"]
+    np.testing.assert_array_equal(ids, sweep.solve_ids)
+    Kr = o["K"].tocsr()[ids][:, ids].tocsr()
+    n = ids.shape[0]
+    sval = np.empty(sweep.n_sentries)
+    sweep.ctx.call("emg_surface_get", ptr(sval))
+    E, B = sweep.ctx.values_get()
+    full_ip, full_ix = sweep.dm.pattern
+    N = sweep.basis.n_field
+    Er = csr_matrix((E, full_ix, full_ip), shape=(N, N))[ids][:, ids]
+    Br = csr_matrix((B, full_ix, full_ip), shape=(N, N))[ids][:, ids]
+    k0 = o["k0"]
+    gam = np.array([b.get_gamma(k0) for b in sweep.robin])
+    rows = np.repeat(np.arange(n), np.diff(sweep.s_ptr))
+    S = csr_matrix((gam[sweep.s_bc] * sval, (rows, sweep.s_col)), shape=(n, n))
+    A = (Er - k0 ** 2 * Br + S).tocsr()
+    D = (A - Kr).tocsr()
+    assert np.abs(D.data).max(initial=0) <= 1e-12 * np.abs(Kr.data).max()
+    # the reduced pattern holds every entry of the oracle's reduced matrix
+    ip, ix = sweep.reduced_pattern()
+    P = csr_matrix((np.ones(ix.shape[0]), ix, ip), shape=(n, n))
+    assert (abs(Kr).sign() - abs(Kr).sign().multiply(P)).nnz == 0
+
+
+def test_forcing_matches_oracle(sweep):
+    o = wg_case.oracle_solution(F0)
+    fv = sweep.forcing()
+    ids = o["ids"]
+    for j, p in enumerate(sweep.ports):
+        ref = o["pv"][p.port_number][ids] / p.uinc_factor(o["k0"])
+        np.testing.assert_allclose(fv[j], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sweep_single_frequency_matches_superlu(sweep, precision):
+    print(f"sweep precision={precision} f={F0}", flush=True)
+    o = wg_case.oracle_solution(F0)
+    res = sweep.solve([F0], precision=precision, tol=1e-8, fblock=2, restart=100, max_cycles=400)
+    ids = o["ids"]
+    K = o["K"].tocsr()
+    for p in sweep.ports:
+        x = res.fields[(0, p.port_number)]
+        b = o["b"] + o["pv"][p.port_number]
+        rr = np.linalg.norm((b - K @ x)[ids]) / np.linalg.norm(b[ids])
+        assert rr <= 1e-8, (p.port_number, rr)
+    fields = {p.port_number: res.fields[(0, p.port_number)] for p in sweep.ports}
+    S = wg_case.s_from_fields(fields, F0)
+    assert np.abs(S - o["S"]).max() <= 1e-6, np.abs(S - o["S"]).max()
+    print(f"iters={res.iters.tolist()} relres={res.relres.tolist()} dS={np.abs(S - o['S']).max():.2e}")
+
+
+def test_sweep_batched_mixed_several_frequencies(sweep):
+    freqs = np.array([8e9, 9.5e9, 11e9, 12e9])
+    res = sweep.solve(freqs, precision="mixed", tol=1e-8, fblock=8, restart=100, max_cycles=400)
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = wg_case.oracle_solution(float(f))
+        fields = {p.port_number: res.fields[(i, p.port_number)] for p in sweep.ports}
+        S = wg_case.s_from_fields(fields, float(f))
+        assert np.abs(S - o["S"]).max() <= 1e-6, (f, np.abs(S - o["S"]).max())
+
+
+def test_solver_plugin_matches_superlu():
+    o = wg_case.oracle_solution(F0)
+    ids = o["ids"]
+    A = o["K"].tocsc()[np.ix_(ids, ids)]
+    b = (o["b"] + o["pv"][1])[ids]
+    s = SolverHIP("", precision="fp64", tol=1e-10)
+    x, rep = s.solve(A, b, None, False, 0)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-10
+    ref = o["fields"][1][ids]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-6
+    x2, _ = s.solve(A, (o["b"] + o["pv"][2])[ids], None, True, 0)
+    assert np.linalg.norm(x2 - o["fields"][2][ids]) / np.linalg.norm(o["fields"][2][ids]) <= 1e-6
+    sm = SolverHIP("", precision="mixed", tol=1e-10)
+    x3, rep3 = sm.solve(A, b, None, False, 0)
+    assert np.linalg.norm(b - A @ x3) / np.linalg.norm(b) <= 1e-10
+    assert rep3.aux["precision"] == "mixed"

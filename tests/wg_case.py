@@ -1,0 +1,55 @@
+"""WR-90 waveguide case shared by the solver tests (product and oracle sides).
+
+Air-filled 22.86 x 10.16 x 40 mm guide as a Kuhn box, TE10 ports on z = 0 and
+z = L with the local frame x along the broad side, PEC on every other
+exterior triangle.
+"""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from emerge_amd import bc as pbc
+from emerge_amd.workloads import WR90_A, WR90_B, waveguide
+from oracle import curlcurl as cc
+from oracle import fem
+from oracle import ports as oports
+
+L = 40e-3
+
+
+@functools.lru_cache(maxsize=4)
+def case(cells=(11, 5, 18), jitter=0.0):
+    p = waveguide(cells=cells, length=L, jitter=jitter, seed=1)
+    m, f = p.mesh, p.basis
+    t1 = m.plane_tris(2, 0.0)
+    t2 = m.plane_tris(2, L)
+    pec = np.setdiff1d(m.boundary_tris(), np.concatenate([t1, t2]))
+    cs1 = ([1, 0, 0], [0, 1, 0], [0, 0, 1], [WR90_A / 2, WR90_B / 2, 0.0])
+    cs2 = ([1, 0, 0], [0, 1, 0], [0, 0, 1], [WR90_A / 2, WR90_B / 2, L])
+    prod_bcs = [pbc.PEC(pec),
+                pbc.RectangularWaveguide(t1, 1, pbc.CoordinateSystem(*cs1), (WR90_A, WR90_B)),
+                pbc.RectangularWaveguide(t2, 2, pbc.CoordinateSystem(*cs2), (WR90_A, WR90_B))]
+    o_ports = [oports.RectangularWaveguide(t1, 1, oports.CS(*cs1), (WR90_A, WR90_B)),
+               oports.RectangularWaveguide(t2, 2, oports.CS(*cs2), (WR90_A, WR90_B))]
+    E, B = cc.tet_mass_stiffness_matrices(m, f.tet_to_field, cc.isotropic(p.eps_r),
+                                          cc.isotropic(1.0 / p.inv_mu_r))
+    return dict(problem=p, mesh=m, basis=f, pec=pec, prod_bcs=prod_bcs, o_ports=o_ports, E=E, B=B,
+                maps=dict(edge_to_field=f.edge_to_field, tri_to_field=f.tri_to_field))
+
+
+@functools.lru_cache(maxsize=16)
+def oracle_solution(freq: float, cells=(11, 5, 18), jitter=0.0):
+    c = case(cells, jitter)
+    K, b, pv, ids = fem.assemble_freq_matrix(c["mesh"], c["maps"], c["E"], c["B"], c["pec"], c["o_ports"], freq)
+    fields = fem.solve_all_ports(K, b, pv, ids)
+    k0 = 2 * np.pi * freq / fem.C0
+    S = fem.s_parameters(c["mesh"], c["basis"].tet_to_field, fields, c["o_ports"], k0)
+    return dict(K=K, b=b, pv=pv, ids=ids, fields=fields, S=S, k0=k0)
+
+
+def s_from_fields(fields: dict, freq: float, cells=(11, 5, 18), jitter=0.0):
+    c = case(cells, jitter)
+    k0 = 2 * np.pi * freq / fem.C0
+    return fem.s_parameters(c["mesh"], c["basis"].tet_to_field, fields, c["o_ports"], k0)
