@@ -57,6 +57,64 @@ def cpu_baseline(n_tets_sample: int = 100_000) -> dict:
                       f"scipy COO->CSR, {dt:.1f} s"}
 
 
+def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int) -> dict:
+    """WR-90 waveguide (Kuhn 11x5x18, 5,940 tets, 41,118 DOFs), TE10 ports,
+    frequency points from a 201-point 8-12 GHz grid, both ports excited per
+    point.  Plain FP64 GPU path: FP64 GMRES one (frequency, port) system at a
+    time.  Mixed path: complex64 GMRES in FP64 refinement, `fblock` systems per
+    batch.  A point counts once every port reached ||b-Ax||/||b|| <= 1e-8."""
+    from emerge_amd.sweep import FrequencySweep
+    from emerge_amd.workloads import waveguide, waveguide_bcs
+    prob = waveguide()
+    bcs = waveguide_bcs(prob)
+    sw = FrequencySweep(prob.mesh, prob.basis, prob.eps_r, prob.inv_mu_r, bcs)
+    grid = np.linspace(8e9, 12e9, 201)
+    plain_f = grid[np.linspace(0, 200, n_plain).astype(int)]
+    mixed_f = grid[np.linspace(0, 200, n_mixed).astype(int)]
+    sw.solve(plain_f[:1], precision="fp64", fblock=1)  # warm-up (code objects, allocations)
+    t0 = time.perf_counter()
+    rp = sw.solve(plain_f, precision="fp64", fblock=1)
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rm = sw.solve(mixed_f, precision="mixed", fblock=fblock)
+    tm = time.perf_counter() - t0
+    ok_p = int(np.sum(np.all(rp.relres <= 1e-8, axis=1)))
+    ok_m = int(np.sum(np.all(rm.relres <= 1e-8, axis=1)))
+    out = {
+        "workload": "WR-90 22.86x10.16x40 mm, Kuhn 11x5x18 (5,940 tets, 41,118 DOFs), TE10 ports, 8-12 GHz",
+        "fp64_plain": {"points": len(plain_f), "converged": ok_p, "seconds": round(tp, 3),
+                       "freq_points_per_s": round(ok_p / tp, 4),
+                       "mean_iters_per_system": float(rp.iters.mean()), "max_relres": float(rp.relres.max())},
+        "mixed_batched": {"points": len(mixed_f), "converged": ok_m, "seconds": round(tm, 3),
+                          "freq_points_per_s": round(ok_m / tm, 4), "fblock": fblock,
+                          "mean_iters_per_system": float(rm.iters.mean()), "max_relres": float(rm.relres.max())},
+    }
+    out["speedup_mixed_vs_fp64"] = round(out["mixed_batched"]["freq_points_per_s"]
+                                         / max(out["fp64_plain"]["freq_points_per_s"], 1e-30), 3)
+    if cpu_points > 0:
+        from oracle import curlcurl as cc, fem
+        from oracle import ports as oports
+        from emerge_amd.workloads import WR90_A, WR90_B
+        m, f = prob.mesh, prob.basis
+        E, B = cc.tet_mass_stiffness_matrices(m, f.tet_to_field, cc.isotropic(prob.eps_r),
+                                              cc.isotropic(1.0 / prob.inv_mu_r))
+        L = prob.meta["length"]
+        o_ports = [oports.RectangularWaveguide(bcs[1].tri_ids, 1, oports.CS([1, 0, 0], [0, 1, 0], [0, 0, 1],
+                                               [WR90_A / 2, WR90_B / 2, 0.0]), (WR90_A, WR90_B)),
+                   oports.RectangularWaveguide(bcs[2].tri_ids, 2, oports.CS([1, 0, 0], [0, 1, 0], [0, 0, 1],
+                                               [WR90_A / 2, WR90_B / 2, L]), (WR90_A, WR90_B))]
+        maps = dict(edge_to_field=f.edge_to_field, tri_to_field=f.tri_to_field)
+        t0 = time.perf_counter()
+        for fr in grid[np.linspace(0, 200, cpu_points).astype(int)]:
+            K, b, pv, ids = fem.assemble_freq_matrix(m, maps, E, B, bcs[0].tri_ids, o_ports, fr)
+            fem.solve_all_ports(K, b, pv, ids)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"freq_points_per_s": round(cpu_points / tc, 4), "cores": 1, "kind": "port",
+                               "sample": f"{cpu_points} points: oracle per-frequency assembly + scipy SuperLU "
+                                         f"(reference options), {tc:.1f} s"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +122,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=60, help="cube cells per side")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-plain", type=int, default=3, help="FP64 plain-path frequency points")
+    ap.add_argument("--sweep-mixed", type=int, default=32, help="mixed batched frequency points")
+    ap.add_argument("--fblock", type=int, default=16)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-only", action="store_true", help="profile helper: run only the solve sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU sharding is "
@@ -73,6 +136,10 @@ def main() -> int:
     from emerge_amd._lib import Context
     from emerge_amd.assembler import MeshOnDevice
     from emerge_amd.workloads import cube_assembly
+
+    if args.sweep_only:
+        print(json.dumps({"sweep": sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock, 0)}), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     prob = cube_assembly(args.n, seed=0)
@@ -113,6 +180,12 @@ def main() -> int:
         cpu = cpu_baseline()
         print(f"[bench] cpu baseline {cpu['value']:.4f} Mtets/s", file=sys.stderr, flush=True)
 
+    sweep = None
+    if not args.no_sweep:
+        sweep = sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock,
+                            0 if args.no_cpu_baseline else 2)
+        print(f"[bench] sweep {json.dumps(sweep)}", file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -141,6 +214,7 @@ def main() -> int:
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -19,6 +19,7 @@ SOURCES = [
     "emg_assemble.hip",
     "emg_surface.hip",
     "emg_solve.hip",
+    "emg_sparam.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")

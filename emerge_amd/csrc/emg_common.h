@@ -112,6 +112,21 @@ struct emg_ctx {
     int32_t n_forcing = 0;
     emg::DBuf<double> forcing;    // 2 x n_forcing x n_red
 
+    // port probes for S-parameter overlaps (mode-overlap ports)
+    int32_t n_ports = 0;
+    int64_t n_probe_tris = 0;
+    std::vector<int64_t> port_tri_start;  // n_ports + 1
+    emg::DBuf<int32_t> probe_tet;         // owning tet per probe triangle
+    emg::DBuf<double> probe_pts;          // [n][6][3]
+    emg::DBuf<double2> probe_shape;       // [n][6][3]
+    emg::DBuf<double> probe_wa;           // [n][6] quadrature weight x area
+    std::vector<double> port_norm;        // sum w a |shape|^2 per port
+
+    // Krylov method of the solves (emg_solver_config)
+    int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s)
+    int idr_s = 4;
+    double inner_rel32 = 1e-5;    // FP32 inner reduction per refinement step
+
     // generic CSR system handed in by the Solver plug-in
     int64_t g_n = 0, g_nnz = 0;
     emg::DBuf<int32_t> g_ptr, g_idx, g_diag;
@@ -131,6 +146,12 @@ inline int fail(emg_ctx* c, int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace emg
+
+namespace emg {
+// S-parameter port overlaps of the F solutions x ([row][f], reduced rows):
+// out[f * n_ports + p] (defined in emg_sparam.hip)
+int port_overlaps(emg_ctx* c, int F, const double2* x, std::vector<double2>& out);
 }  // namespace emg
 
 #define EMG_HIP(ctx, call)                                                      \

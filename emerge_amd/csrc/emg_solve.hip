@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 #include "emg_common.h"
 #include "emg_cplx.h"
@@ -119,59 +120,66 @@ __global__ void k_jacobi(Op<R> op, const R* __restrict__ k2, const cplx<R>* __re
     dinv[q] = from_d<R>(inv);
 }
 
-// Stage 1 of batched inner products: each wavefront covers 512 consecutive
-// [row][f] elements and writes partial[wave][i][f] = sum conj(V_i) w.
+// Stage 1 of batched inner products: wavefront (chunk, i) with i =
+// blockIdx.y covers 512 consecutive [row][f] elements of vector i and writes
+// partial[i][chunk][f] = sum conj(V_i) w.  All (vector, chunk) pairs run in
+// parallel.
 template <typename R, int F>
-__global__ __launch_bounds__(256) void k_dots(int64_t nq, int nv, const cplx<R>* __restrict__ V, int64_t ldv,
-                                              const cplx<R>* __restrict__ w, double2* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_dots(int64_t nq, int64_t nchunk, const cplx<R>* __restrict__ V,
+                                              int64_t ldv, const cplx<R>* __restrict__ w,
+                                              double2* __restrict__ partial) {
     constexpr int P = 8;
-    const int64_t wave = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const int64_t chunk = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const int i = blockIdx.y;
     const int lane = threadIdx.x & 63;
-    const int64_t base = wave * 64 * P;
-    if (base >= nq) return;
-    double2 wv[P];
-    bool ok[P];
+    if (chunk >= nchunk) return;
+    const int64_t base = chunk * 64 * P;
+    const cplx<R>* Vi = V + i * ldv;
+    double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         const int64_t q = base + p * 64 + lane;
-        ok[p] = q < nq;
-        wv[p] = ok[p] ? to_d(w[q]) : make_double2(0.0, 0.0);
-    }
-    for (int i = 0; i < nv; ++i) {
-        double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            if (!ok[p]) continue;
-            const double2 v = to_d(V[i * ldv + base + p * 64 + lane]);
-            acc.x += v.x * wv[p].x + v.y * wv[p].y;
-            acc.y += v.x * wv[p].y - v.y * wv[p].x;
+        if (q < nq) {
+            const double2 v = to_d(Vi[q]);
+            const double2 x = to_d(w[q]);
+            acc.x += v.x * x.x + v.y * x.y;
+            acc.y += v.x * x.y - v.y * x.x;
         }
-#pragma unroll
-        for (int o = 32; o >= F; o >>= 1) {
-            acc.x += __shfl_xor(acc.x, o);
-            acc.y += __shfl_xor(acc.y, o);
-        }
-        if (lane < F) partial[(wave * nv + i) * F + lane] = acc;
     }
+#pragma unroll
+    for (int o = 32; o >= F; o >>= 1) {
+        acc.x += __shfl_xor(acc.x, o);
+        acc.y += __shfl_xor(acc.y, o);
+    }
+    if (lane < F) partial[(int64_t(i) * nchunk + chunk) * F + lane] = acc;
 }
 
-// Stage 2: out[i][f] (+)= sum over waves in order.
-__global__ void k_dots_final(int64_t nwaves, int nv, int F, const double2* __restrict__ partial,
+// Stage 2: out[i][f] (+)= sum over chunks in order; one wavefront per (i, f)
+// group of 64 chunks strided, then a fixed-order tree.
+__global__ void k_dots_final(int64_t nchunk, int nv, int F, const double2* __restrict__ partial,
                              double2* __restrict__ out, int accumulate) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x;  // (i, f)
     if (t >= nv * F) return;
     const int i = t / F, f = t % F;
+    const int lane = threadIdx.x;  // 64 lanes
     double2 s = make_double2(0.0, 0.0);
-    for (int64_t wv = 0; wv < nwaves; ++wv) {
-        const double2 v = partial[(wv * nv + i) * F + f];
+    for (int64_t c = lane; c < nchunk; c += 64) {
+        const double2 v = partial[(int64_t(i) * nchunk + c) * F + f];
         s.x += v.x;
         s.y += v.y;
     }
-    if (accumulate) {
-        out[t].x += s.x;
-        out[t].y += s.y;
-    } else {
-        out[t] = s;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        s.x += __shfl_xor(s.x, o);
+        s.y += __shfl_xor(s.y, o);
+    }
+    if (lane == 0) {
+        if (accumulate) {
+            out[t].x += s.x;
+            out[t].y += s.y;
+        } else {
+            out[t] = s;
+        }
     }
 }
 
@@ -183,6 +191,7 @@ __global__ void k_axpy_multi(int64_t nq, int nv, const cplx<R>* __restrict__ V, 
     if (q >= nq) return;
     const int f = int(q % F);
     double2 acc = to_d(w[q]);
+#pragma unroll 8
     for (int i = 0; i < nv; ++i) {
         const double2 v = to_d(V[i * ldv + q]);
         const double2 c = h[i * F + f];
@@ -289,6 +298,7 @@ __global__ void k_update(int64_t nq, int k, const cplx<R>* __restrict__ V, int64
     if (q >= nq) return;
     const int f = int(q % F);
     double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 8
     for (int i = 0; i < k; ++i) {
         const double2 v = to_d(V[i * ldv + q]);
         const double2 c = y[i * F + f];
@@ -353,10 +363,12 @@ __global__ void k_transpose(int64_t n, int F, const double2* __restrict__ in, do
 
 struct SolveParams {
     double tol = 1e-8;
-    int restart = 100;
+    int restart = 100;        // GMRES: Krylov dimension; IDR: matvecs per refinement cycle
     int max_cycles = 100;
     double inner_rel = 1e-5;  // inner (per-cycle) reduction target for FP32
     int check_every = 10;
+    int method = 0;           // 0 = GMRES(m), 1 = IDR(s)
+    int idr_s = 4;
 };
 
 struct BatchStats {
@@ -398,8 +410,8 @@ int run_batch(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double*
     EMG_HIP(c, W.y.alloc(int64_t(m) * F));
     EMG_HIP(c, W.nrm.alloc(F));
     EMG_HIP(c, W.res.alloc(F));
-    const int64_t nwaves = (nq + 511) / 512;
-    EMG_HIP(c, W.partial.alloc(nwaves * (m + 1) * F));
+    const int64_t nchunk = (nq + 511) / 512;
+    EMG_HIP(c, W.partial.alloc(nchunk * (m + 1) * F));
     EMG_HIP(c, W.k2.alloc(F));
     EMG_HIP(c, W.k264.alloc(F));
     EMG_HIP(c, W.gam.alloc(std::max(1, nbc) * F));
@@ -422,17 +434,17 @@ int run_batch(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double*
     }
     const unsigned gq = grid_for(nq, 256);
     const unsigned grow = grid_for(n, 4);
-    const unsigned gdots = grid_for(nwaves * 64, 256);
+    const unsigned gdots = grid_for(nchunk * 64, 256);
     k_jacobi<R, F><<<gq, 256, 0, s>>>(op, W.k2.p, W.gam.p, W.dinv.p);
     EMG_HIP(c, hipGetLastError());
 
     auto dots = [&](const cplx<R>* V, int nv, const cplx<R>* w, double2* out, int acc) {
-        k_dots<R, F><<<gdots, 256, 0, s>>>(nq, nv, V, nq, w, W.partial.p);
-        k_dots_final<<<grid_for(int64_t(nv) * F, 64), 64, 0, s>>>(nwaves, nv, F, W.partial.p, out, acc);
+        k_dots<R, F><<<dim3(gdots, nv), 256, 0, s>>>(nq, nchunk, V, nq, w, W.partial.p);
+        k_dots_final<<<nv * F, 64, 0, s>>>(nchunk, nv, F, W.partial.p, out, acc);
     };
     auto dots64 = [&](const double2* a, double2* out) {
-        k_dots<double, F><<<gdots, 256, 0, s>>>(nq, 1, a, nq, a, W.partial.p);
-        k_dots_final<<<grid_for(F, 64), 64, 0, s>>>(nwaves, 1, F, W.partial.p, out, 0);
+        k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, a, W.partial.p);
+        k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, W.partial.p, out, 0);
     };
 
     // ||b||
@@ -484,7 +496,7 @@ int run_batch(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double*
             // second pass into y (scratch), accumulated onto hbuf
             dots(W.V.p, j + 1, W.w.p, W.y.p, 0);
             k_axpy_multi<R, F><<<gq, 256, 0, s>>>(nq, j + 1, W.V.p, nq, W.y.p, W.w.p);
-            k_dots_final<<<grid_for(int64_t(j + 1) * F, 64), 64, 0, s>>>(nwaves, j + 1, F, W.partial.p, W.hbuf.p, 1);
+            k_dots_final<<<(j + 1) * F, 64, 0, s>>>(nchunk, j + 1, F, W.partial.p, W.hbuf.p, 1);
             dots(W.w.p, 1, W.w.p, W.nrm.p, 0);
             k_scale<R, F><<<gq, 256, 0, s>>>(nq, W.w.p, W.nrm.p, W.V.p + int64_t(j + 1) * nq);
             k_givens<<<1, 64, 0, s>>>(j, m, F, W.hbuf.p, W.nrm.p, W.H.p, W.cs.p, W.sn.p, W.g.p, W.res.p);
@@ -508,9 +520,360 @@ int run_batch(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double*
     return EMG_OK;
 }
 
+// ------------------------------------------------------------------ IDR(s)
+// Induced dimension reduction IDR(s) with bi-orthogonalisation (van Gijzen &
+// Sonneveld 2011, "Algorithm 913"), right Jacobi scaling and the
+// "maintaining the convergence" choice of omega (threshold 0.7).  Short
+// recurrences: per matrix-vector product O(s) vector updates, independent of
+// the iteration count.
+
+// w = r - sum_{j<cnt} c[j][f] * G_{k+j}; w = dinv o w  (v of Alg. 913)
+template <typename R, int F>
+__global__ void k_idr_v(int64_t nq, int k, int cnt, const cplx<R>* __restrict__ G, int64_t ldv,
+                        const double2* __restrict__ c, const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ dinv,
+                        cplx<R>* __restrict__ v) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int f = int(q % F);
+    double2 a = to_d(r[q]);
+    for (int j = 0; j < cnt; ++j) {
+        const double2 g = to_d(G[(k + j) * ldv + q]);
+        const double2 cj = c[j * F + f];
+        a.x -= cj.x * g.x - cj.y * g.y;
+        a.y -= cj.x * g.y + cj.y * g.x;
+    }
+    const double2 d = to_d(dinv[q]);
+    v[q] = from_d<R>(make_double2(d.x * a.x - d.y * a.y, d.x * a.y + d.y * a.x));
+}
+
+// U_k = sum_{j<cnt} c[j][f] U_{k+j} + om[f] v   (in place for slot k)
+template <typename R, int F>
+__global__ void k_idr_u(int64_t nq, int k, int cnt, cplx<R>* __restrict__ U, int64_t ldv,
+                        const double2* __restrict__ c, const double2* __restrict__ om, const cplx<R>* __restrict__ v) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int f = int(q % F);
+    const double2 o = om[f];
+    const double2 vv = to_d(v[q]);
+    double2 a = make_double2(o.x * vv.x - o.y * vv.y, o.x * vv.y + o.y * vv.x);
+    for (int j = 0; j < cnt; ++j) {
+        const double2 u = to_d(U[(k + j) * ldv + q]);
+        const double2 cj = c[j * F + f];
+        a.x += cj.x * u.x - cj.y * u.y;
+        a.y += cj.x * u.y + cj.y * u.x;
+    }
+    U[k * ldv + q] = from_d<R>(a);
+}
+
+// G_k -= sum_{i<k} a[i][f] G_i ; U_k -= sum_{i<k} a[i][f] U_i
+template <typename R, int F>
+__global__ void k_idr_biorth(int64_t nq, int k, cplx<R>* __restrict__ G, cplx<R>* __restrict__ U, int64_t ldv,
+                             const double2* __restrict__ a) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int f = int(q % F);
+    double2 g = to_d(G[k * ldv + q]), u = to_d(U[k * ldv + q]);
+    for (int i = 0; i < k; ++i) {
+        const double2 ai = a[i * F + f];
+        const double2 gi = to_d(G[i * ldv + q]), ui = to_d(U[i * ldv + q]);
+        g.x -= ai.x * gi.x - ai.y * gi.y;
+        g.y -= ai.x * gi.y + ai.y * gi.x;
+        u.x -= ai.x * ui.x - ai.y * ui.y;
+        u.y -= ai.x * ui.y + ai.y * ui.x;
+    }
+    G[k * ldv + q] = from_d<R>(g);
+    U[k * ldv + q] = from_d<R>(u);
+}
+
+// r -= beta G_k ; x64 += beta U_k   (beta per system)
+template <typename R, int F>
+__global__ void k_idr_rx(int64_t nq, const cplx<R>* __restrict__ Gk, const cplx<R>* __restrict__ Uk,
+                         const double2* __restrict__ beta, cplx<R>* __restrict__ r, double2* __restrict__ x) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int f = int(q % F);
+    const double2 b = beta[f];
+    const double2 g = to_d(Gk[q]), u = to_d(Uk[q]);
+    double2 rr = to_d(r[q]);
+    rr.x -= b.x * g.x - b.y * g.y;
+    rr.y -= b.x * g.y + b.y * g.x;
+    r[q] = from_d<R>(rr);
+    x[q].x += b.x * u.x - b.y * u.y;
+    x[q].y += b.x * u.y + b.y * u.x;
+}
+
+// x64 += om (dinv o r) ; r -= om t
+template <typename R, int F>
+__global__ void k_idr_omega_update(int64_t nq, const double2* __restrict__ om, const cplx<R>* __restrict__ dinv,
+                                   const cplx<R>* __restrict__ t, cplx<R>* __restrict__ r, double2* __restrict__ x) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int f = int(q % F);
+    const double2 o = om[f];
+    const double2 rv = to_d(r[q]), d = to_d(dinv[q]), tv = to_d(t[q]);
+    const double2 v = make_double2(d.x * rv.x - d.y * rv.y, d.x * rv.y + d.y * rv.x);
+    x[q].x += o.x * v.x - o.y * v.y;
+    x[q].y += o.x * v.y + o.y * v.x;
+    r[q] = from_d<R>(make_double2(rv.x - (o.x * tv.x - o.y * tv.y), rv.y - (o.x * tv.y + o.y * tv.x)));
+}
+
+__device__ __forceinline__ double2 cdivd(double2 a, double2 b) {
+    const double m = b.x * b.x + b.y * b.y;
+    if (!(m > 0)) return make_double2(0.0, 0.0);
+    return make_double2((a.x * b.x + a.y * b.y) / m, (a.y * b.x - a.x * b.y) / m);
+}
+
+// forward substitution: c = M[k:,k:]^-1 fvec[k:]  (M lower triangular,
+// M[i][j] at (i*s + j)*F + f)
+__global__ void k_idr_solve_c(int k, int s, int F, const double2* __restrict__ M, const double2* __restrict__ fv,
+                              double2* __restrict__ c) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    for (int j = 0; j < s - k; ++j) {
+        double2 a = fv[(k + j) * F + f];
+        for (int i = 0; i < j; ++i) {
+            const double2 m = M[((k + j) * s + (k + i)) * F + f];
+            const double2 ci = c[i * F + f];
+            a.x -= m.x * ci.x - m.y * ci.y;
+            a.y -= m.x * ci.y + m.y * ci.x;
+        }
+        c[j * F + f] = cdivd(a, M[((k + j) * s + (k + j)) * F + f]);
+    }
+}
+
+// biorthogonalisation coefficients from d_l = p_l^H g_k (l < k):
+// a_l = (d_l - sum_{i<l} M[l][i] a_i) / M[l][l]
+__global__ void k_idr_alpha(int k, int s, int F, const double2* __restrict__ M, const double2* __restrict__ d,
+                            double2* __restrict__ a) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    for (int l = 0; l < k; ++l) {
+        double2 v = d[l * F + f];
+        for (int i = 0; i < l; ++i) {
+            const double2 m = M[(l * s + i) * F + f];
+            const double2 ai = a[i * F + f];
+            v.x -= m.x * ai.x - m.y * ai.y;
+            v.y -= m.x * ai.y + m.y * ai.x;
+        }
+        a[l * F + f] = cdivd(v, M[(l * s + l) * F + f]);
+    }
+}
+
+// column k of M from d_j = p_j^H g_k (all j); beta = f_k / M_kk;
+// f[j] -= beta M[j][k] for j > k
+__global__ void k_idr_column(int k, int s, int F, const double2* __restrict__ d, double2* __restrict__ M,
+                             double2* __restrict__ fv, double2* __restrict__ beta) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    for (int j = k; j < s; ++j) M[(j * s + k) * F + f] = d[j * F + f];
+    const double2 b = cdivd(fv[k * F + f], M[(k * s + k) * F + f]);
+    beta[f] = b;
+    for (int j = k + 1; j < s; ++j) {
+        const double2 m = M[(j * s + k) * F + f];
+        fv[j * F + f].x -= b.x * m.x - b.y * m.y;
+        fv[j * F + f].y -= b.x * m.y + b.y * m.x;
+    }
+}
+
+// om = (t^H r) / (t^H t), enlarged to keep |cos| >= 0.7; tt / tr / rr given
+// as dot results ("maintaining the convergence", Sleijpen & van der Vorst)
+__global__ void k_idr_omega(int F, const double2* __restrict__ tt, const double2* __restrict__ rt,
+                            const double2* __restrict__ rr, double2* __restrict__ om) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const double ntt = tt[f].x, nrr = rr[f].x;
+    const double2 tr = make_double2(rt[f].x, -rt[f].y);  // t^H r = conj(r^H t)
+    double2 o = ntt > 0 ? make_double2(tr.x / ntt, tr.y / ntt) : make_double2(0.0, 0.0);
+    const double denom = sqrt(fmax(ntt, 0.0)) * sqrt(fmax(nrr, 0.0));
+    const double rho = denom > 0 ? sqrt(tr.x * tr.x + tr.y * tr.y) / denom : 1.0;
+    if (rho < 0.7 && rho > 0) {
+        o.x *= 0.7 / rho;
+        o.y *= 0.7 / rho;
+    }
+    om[f] = o;
+}
+
+__global__ void k_set_c(int64_t n, double2* __restrict__ p, double2 v) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+template <typename R, int F>
+int run_idr(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k2_h, const double2* gam_h, int nbc,
+            const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
+    hipStream_t s = c->stream;
+    const int64_t n = op.n, nq = n * F;
+    const int S = P.idr_s;
+    DBuf<cplx<R>> Pm, G, U, r, v, t, dinv;
+    DBuf<double2> r64, t64, Mm, fv, cvec, avec, beta, om, dbuf, tt, rt, rr, partial, bn;
+    EMG_HIP(c, Pm.alloc(int64_t(S) * nq));
+    EMG_HIP(c, G.alloc(int64_t(S) * nq));
+    EMG_HIP(c, U.alloc(int64_t(S) * nq));
+    EMG_HIP(c, r.alloc(nq));
+    EMG_HIP(c, v.alloc(nq));
+    EMG_HIP(c, t.alloc(2 * nq));  // t followed by a copy slot for r (dots of t against [t, r])
+    EMG_HIP(c, dinv.alloc(nq));
+    EMG_HIP(c, r64.alloc(nq));
+    EMG_HIP(c, t64.alloc(nq));
+    EMG_HIP(c, Mm.alloc(int64_t(S) * S * F));
+    EMG_HIP(c, fv.alloc(int64_t(S) * F));
+    EMG_HIP(c, cvec.alloc(int64_t(S) * F));
+    EMG_HIP(c, avec.alloc(int64_t(S) * F));
+    EMG_HIP(c, beta.alloc(F));
+    EMG_HIP(c, om.alloc(F));
+    EMG_HIP(c, dbuf.alloc(int64_t(S + 2) * F));
+    EMG_HIP(c, tt.alloc(2 * F));
+    EMG_HIP(c, rt.alloc(F));
+    EMG_HIP(c, rr.alloc(F));
+    EMG_HIP(c, bn.alloc(F));
+    const int64_t nchunk = (nq + 511) / 512;
+    EMG_HIP(c, partial.alloc(nchunk * (S + 2) * F));
+    DBuf<R> k2d;
+    DBuf<double> k264;
+    DBuf<cplx<R>> gam;
+    DBuf<double2> gam64;
+    EMG_HIP(c, k2d.alloc(F));
+    EMG_HIP(c, k264.alloc(F));
+    EMG_HIP(c, gam.alloc(std::max(1, nbc) * F));
+    EMG_HIP(c, gam64.alloc(std::max(1, nbc) * F));
+    {
+        std::vector<R> k2r(F);
+        std::vector<cplx<R>> gr(std::max(1, nbc) * F, cmk<R>(R(0), R(0)));
+        std::vector<double2> g64(std::max(1, nbc) * F, make_double2(0, 0));
+        for (int f = 0; f < F; ++f) k2r[f] = R(k2_h[f]);
+        for (int b = 0; b < nbc; ++b)
+            for (int f = 0; f < F; ++f) {
+                gr[b * F + f] = from_d<R>(gam_h[b * F + f]);
+                g64[b * F + f] = gam_h[b * F + f];
+            }
+        EMG_HIP(c, hipMemcpyAsync(k2d.p, k2r.data(), sizeof(R) * F, hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipMemcpyAsync(k264.p, k2_h, sizeof(double) * F, hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipMemcpyAsync(gam.p, gr.data(), sizeof(cplx<R>) * gr.size(), hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipMemcpyAsync(gam64.p, g64.data(), sizeof(double2) * g64.size(), hipMemcpyHostToDevice, s));
+        // shadow space: fixed pseudo-random complex vectors, same for every system
+        std::vector<cplx<R>> ph(size_t(S) * nq);
+        uint64_t z = 0x9E3779B97F4A7C15ull;
+        auto next = [&]() {
+            z ^= z << 13;
+            z ^= z >> 7;
+            z ^= z << 17;
+            return double(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        };
+        for (int i = 0; i < S; ++i)
+            for (int64_t row = 0; row < n; ++row) {
+                const double a = next(), b = next();
+                for (int f = 0; f < F; ++f) ph[size_t(i) * nq + row * F + f] = cmk<R>(R(a), R(b));
+            }
+        EMG_HIP(c, hipMemcpyAsync(Pm.p, ph.data(), sizeof(cplx<R>) * ph.size(), hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipStreamSynchronize(s));
+    }
+    const unsigned gq = grid_for(nq, 256);
+    const unsigned grow = grid_for(n, 4);
+    const unsigned gdots = grid_for(nchunk * 64, 256);
+    k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
+    EMG_HIP(c, hipGetLastError());
+    auto dots = [&](const cplx<R>* V, int nv, const cplx<R>* w, double2* out) {
+        k_dots<R, F><<<dim3(gdots, nv), 256, 0, s>>>(nq, nchunk, V, nq, w, partial.p);
+        k_dots_final<<<nv * F, 64, 0, s>>>(nchunk, nv, F, partial.p, out, 0);
+    };
+    auto dots64 = [&](const double2* a, double2* out) {
+        k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, a, partial.p);
+        k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
+    };
+    std::vector<double2> hb(F), hr(F);
+    dots64(b64, bn.p);
+    EMG_HIP(c, hipMemcpyAsync(hb.data(), bn.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipStreamSynchronize(s));
+    std::vector<double> bnorm(F);
+    for (int f = 0; f < F; ++f) bnorm[f] = std::sqrt(std::max(hb[f].x, 0.0));
+    st.relres.assign(F, 1.0);
+    st.iters.assign(F, 0);
+    const int64_t max_mv = int64_t(P.restart) * P.max_cycles;  // total matvec budget
+    int64_t mv = 0;
+    for (int cyc = 0; cyc < P.max_cycles && mv < max_mv; ++cyc) {
+        k_spmv<double, F><<<grow, 256, 0, s>>>(op64, k264.p, gam64.p, x64, nullptr, t64.p);
+        k_sub<<<gq, 256, 0, s>>>(nq, b64, t64.p, r64.p);
+        dots64(r64.p, rr.p);
+        EMG_HIP(c, hipMemcpyAsync(hr.data(), rr.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+        EMG_HIP(c, hipStreamSynchronize(s));
+        EMG_HIP(c, hipGetLastError());
+        bool all = true;
+        std::vector<double> target(F);
+        for (int f = 0; f < F; ++f) {
+            const double rn = std::sqrt(std::max(hr[f].x, 0.0));
+            st.relres[f] = bnorm[f] > 0 ? rn / bnorm[f] : 0.0;
+            if (!(st.relres[f] <= P.tol)) all = false;
+            target[f] = std::max(0.5 * P.tol * bnorm[f], P.inner_rel * rn);
+        }
+        st.cycles = cyc;
+        if (all) break;
+        // inner IDR(s) on A d = r64, accumulating d into x64
+        k_cast<R><<<gq, 256, 0, s>>>(nq, r64.p, r.p);
+        EMG_HIP(c, hipMemsetAsync(G.p, 0, sizeof(cplx<R>) * S * nq, s));
+        EMG_HIP(c, hipMemsetAsync(U.p, 0, sizeof(cplx<R>) * S * nq, s));
+        {
+            std::vector<double2> eye(size_t(S) * S * F, make_double2(0, 0));
+            for (int i = 0; i < S; ++i)
+                for (int f = 0; f < F; ++f) eye[(size_t(i) * S + i) * F + f] = make_double2(1, 0);
+            EMG_HIP(c, hipMemcpyAsync(Mm.p, eye.data(), sizeof(double2) * eye.size(), hipMemcpyHostToDevice, s));
+        }
+        k_set_c<<<1, 64, 0, s>>>(F, om.p, make_double2(1.0, 0.0));
+        int64_t inner = 0;
+        bool done = false;
+        while (!done && inner < P.restart && mv < max_mv) {
+            dots(Pm.p, S, r.p, fv.p);  // f = P^H r
+            for (int k = 0; k < S; ++k) {
+                k_idr_solve_c<<<1, 64, 0, s>>>(k, S, F, Mm.p, fv.p, cvec.p);
+                k_idr_v<R, F><<<gq, 256, 0, s>>>(nq, k, S - k, G.p, nq, cvec.p, r.p, dinv.p, v.p);
+                k_idr_u<R, F><<<gq, 256, 0, s>>>(nq, k, S - k, U.p, nq, cvec.p, om.p, v.p);
+                k_spmv<R, F><<<grow, 256, 0, s>>>(op, k2d.p, gam.p, U.p + int64_t(k) * nq, nullptr,
+                                                   G.p + int64_t(k) * nq);
+                if (k > 0) {
+                    dots(Pm.p, k, G.p + int64_t(k) * nq, dbuf.p);
+                    k_idr_alpha<<<1, 64, 0, s>>>(k, S, F, Mm.p, dbuf.p, avec.p);
+                    k_idr_biorth<R, F><<<gq, 256, 0, s>>>(nq, k, G.p, U.p, nq, avec.p);
+                }
+                dots(Pm.p, S, G.p + int64_t(k) * nq, dbuf.p);
+                k_idr_column<<<1, 64, 0, s>>>(k, S, F, dbuf.p, Mm.p, fv.p, beta.p);
+                k_idr_rx<R, F><<<gq, 256, 0, s>>>(nq, G.p + int64_t(k) * nq, U.p + int64_t(k) * nq, beta.p, r.p, x64);
+                ++inner;
+                ++mv;
+            }
+            // omega step: t = A (dinv o r)
+            k_spmv<R, F><<<grow, 256, 0, s>>>(op, k2d.p, gam.p, r.p, dinv.p, t.p);
+            EMG_HIP(c, hipMemcpyAsync(t.p + nq, r.p, sizeof(cplx<R>) * nq, hipMemcpyDeviceToDevice, s));
+            dots(t.p, 2, t.p, tt.p);      // [t^H t, r^H t]
+            dots(r.p, 1, r.p, rr.p);      // r^H r
+            k_idr_omega<<<1, 64, 0, s>>>(F, tt.p, tt.p + F, rr.p, om.p);
+            k_idr_omega_update<R, F><<<gq, 256, 0, s>>>(nq, om.p, dinv.p, t.p, r.p, x64);
+            ++inner;
+            ++mv;
+            // recursive residual check
+            dots(r.p, 1, r.p, rr.p);
+            EMG_HIP(c, hipMemcpyAsync(hr.data(), rr.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipStreamSynchronize(s));
+            EMG_HIP(c, hipGetLastError());
+            done = true;
+            for (int f = 0; f < F; ++f)
+                if (!(std::sqrt(std::max(hr[f].x, 0.0)) <= target[f])) done = false;
+        }
+        for (int f = 0; f < F; ++f) st.iters[f] += inner;
+        st.cycles = cyc + 1;
+    }
+    return EMG_OK;
+}
+
+template <typename R, int F>
+int run_method(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k2, const double2* gam, int nbc,
+               const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
+    if (P.method == 1) return run_idr<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
+    return run_batch<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
+}
+
 template <typename R>
 int dispatch_batch(emg_ctx* c, int F, const Op<R>& op, const Op<double>& op64, const double* k2, const double2* gam,
                    int nbc, const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
+#define run_batch run_method
     switch (F) {
         case 1: return run_batch<R, 1>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
         case 2: return run_batch<R, 2>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
@@ -520,6 +883,7 @@ int dispatch_batch(emg_ctx* c, int F, const Op<R>& op, const Op<double>& op64, c
         case 32: return run_batch<R, 32>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
         default: return fail(c, EMG_ERR_ARG, "batch width %d not in {1,2,4,8,16,32}", F);
     }
+#undef run_batch
 }
 
 int pow2_block(int64_t remaining, int fblock) {
@@ -532,12 +896,14 @@ int pow2_block(int64_t remaining, int fblock) {
 
 extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, const double* gamma,
                                const double* rhs_coef, int precision, int fblock, double tol, int restart,
-                               int max_cycles, double* x_out, double* relres_out, int64_t* iters_out) {
+                               int max_cycles, double* x_out, double* relres_out, int64_t* iters_out,
+                               double* overlap_out) {
     EMG_CHECK_CTX(c);
     if (!c->have_system) return fail(c, EMG_ERR_STATE, "emg_sweep_solve: no reduced system");
+    if (overlap_out && c->n_ports == 0) return fail(c, EMG_ERR_STATE, "emg_sweep_solve: no port probes set");
     if (!c->have_surface) return fail(c, EMG_ERR_STATE, "emg_sweep_solve: surface terms not set");
     if (c->n_forcing < 1) return fail(c, EMG_ERR_STATE, "emg_sweep_solve: no forcing vectors");
-    if (nbatch < 1 || !k2 || !rhs_coef || !x_out || (c->n_bc > 0 && !gamma))
+    if (nbatch < 1 || !k2 || !rhs_coef || (c->n_bc > 0 && !gamma))
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: bad arguments");
     if (restart < 1 || restart > 1000 || max_cycles < 1 || !(tol > 0))
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: bad solver parameters");
@@ -576,7 +942,9 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     P.tol = tol;
     P.restart = restart;
     P.max_cycles = max_cycles;
-    P.inner_rel = precision == 1 ? 1e-5 : 1e-13;
+    P.inner_rel = precision == 1 ? c->inner_rel32 : 1e-13;
+    P.method = c->krylov_method;
+    P.idr_s = c->idr_s;
     DBuf<double2> b64, x64, xt, coef;
     int64_t done = 0;
     while (done < nbatch) {
@@ -598,8 +966,16 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
                            ? dispatch_batch<float>(c, F, op32, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st)
                            : dispatch_batch<double>(c, F, op64, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st);
         if (rc != EMG_OK) return rc;
-        k_transpose<<<grid_for(n * F, 256), 256, 0, s>>>(n, F, x64.p, xt.p, 1);
-        EMG_HIP(c, hipMemcpyAsync(x_out + 2 * done * n, xt.p, sizeof(double2) * n * F, hipMemcpyDeviceToHost, s));
+        if (overlap_out) {
+            std::vector<double2> ov;
+            const int orc = port_overlaps(c, F, x64.p, ov);
+            if (orc != EMG_OK) return orc;
+            std::memcpy(overlap_out + 2 * done * c->n_ports, ov.data(), sizeof(double2) * ov.size());
+        }
+        if (x_out) {
+            k_transpose<<<grid_for(n * F, 256), 256, 0, s>>>(n, F, x64.p, xt.p, 1);
+            EMG_HIP(c, hipMemcpyAsync(x_out + 2 * done * n, xt.p, sizeof(double2) * n * F, hipMemcpyDeviceToHost, s));
+        }
         EMG_HIP(c, hipStreamSynchronize(s));
         for (int f = 0; f < F; ++f) {
             if (relres_out) relres_out[done + f] = st.relres[f];
@@ -675,7 +1051,9 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
     P.tol = tol;
     P.restart = restart;
     P.max_cycles = max_cycles;
-    P.inner_rel = precision == 1 ? 1e-5 : 1e-13;
+    P.inner_rel = precision == 1 ? c->inner_rel32 : 1e-13;
+    P.method = c->krylov_method;
+    P.idr_s = c->idr_s;
     DBuf<double2> bin, b64, x64, xt;
     int64_t done = 0;
     int rc = EMG_OK;
@@ -706,5 +1084,16 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
         done += F;
     }
     if (!converged) return fail(c, EMG_ERR_CONVERGE, "emg_csr_solve: tolerance %.1e not reached", tol);
+    return EMG_OK;
+}
+
+extern "C" int emg_solver_config(emg_ctx* c, int method, int idr_s, double inner_rel32) {
+    EMG_CHECK_CTX(c);
+    if (method != 0 && method != 1) return fail(c, EMG_ERR_ARG, "emg_solver_config: method must be 0 (GMRES) or 1 (IDR)");
+    if (idr_s < 1 || idr_s > 16) return fail(c, EMG_ERR_ARG, "emg_solver_config: idr_s must be in [1, 16]");
+    if (!(inner_rel32 > 0 && inner_rel32 < 1)) return fail(c, EMG_ERR_ARG, "emg_solver_config: inner_rel32 in (0,1)");
+    c->krylov_method = method;
+    c->idr_s = idr_s;
+    c->inner_rel32 = inner_rel32;
     return EMG_OK;
 }

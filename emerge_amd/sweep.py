@@ -47,11 +47,30 @@ def dunavant_points(nodes: np.ndarray, tris: np.ndarray, tri_ids: np.ndarray) ->
 class SweepResult:
     freqs: np.ndarray
     ports: list
-    fields: dict            # (freq index, port number) -> full solution vector
+    fields: dict            # (freq index, port number) -> full solution vector (if kept)
     relres: np.ndarray      # (nfreq, nports)
     iters: np.ndarray
     precision: str
+    S: np.ndarray | None = None   # (nfreq, nports, nports), rows = receiving port
     timing: dict = field(default_factory=dict)
+
+
+def apply_s_caps(Sp: np.ndarray, cap_single: bool = True, cap_col: bool = True, recip: bool = False):
+    """Post-corrections of MW/microwave_3d.py:1223-1235 (defaults of
+    E/settings.py:25-27): reciprocity average, column power cap (divides by
+    the column power), single-entry cap."""
+    Sp = Sp.copy()
+    n = Sp.shape[0]
+    if recip:
+        Sp = (Sp + Sp.T) / 2
+    if cap_col:
+        for j in range(n):
+            Sp[:, j] = Sp[:, j] / max(1.0, np.sum(np.abs(Sp[:, j]) ** 2))
+    if cap_single:
+        for i in range(n):
+            for j in range(n):
+                Sp[i, j] = Sp[i, j] / max(1.0, np.abs(Sp[i, j]))
+    return Sp
 
 
 class FrequencySweep:
@@ -174,6 +193,34 @@ class FrequencySweep:
             self.ctx.call("emg_forcing_set", len(self.ports), frow.shape[0], ptr(frow), ptr(fseg),
                           fcon_a.shape[0], ptr(fcon_a))
 
+        # S-parameter probes: mode-overlap ports (the lumped-port voltage line
+        # integral is not implemented on the device)
+        self.port_norms = None
+        if self.ports and all(isinstance(p, bcs.RectangularWaveguide) for p in self.ports):
+            counts, owners, P_, SH, WA = [], [], [], [], []
+            for p in self.ports:
+                t = p.tri_ids
+                counts.append(t.shape[0])
+                owners.append(m.tri_to_tet[0, t])
+                pts = dunavant_points(m.nodes, m.tris, t)
+                P_.append(pts)
+                shp = np.asarray(p.mode_shape(pts[..., 0].reshape(-1), pts[..., 1].reshape(-1),
+                                              pts[..., 2].reshape(-1)), dtype=np.complex128)
+                SH.append(shp.reshape(3, -1).T.reshape(-1, 6, 3))
+                v = m.nodes[:, m.tris[:, t]]
+                cr = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], axis=0)
+                area = np.sqrt(cr[0] ** 2 + cr[1] ** 2 + cr[2] ** 2) / 2  # calc_area, E/mth/optimized.py:276-291
+                WA.append(_DUN_W[None, :] * area[:, None])
+            cnt = np.ascontiguousarray(counts, dtype=np.int64)
+            own = np.ascontiguousarray(np.concatenate(owners), dtype=np.int32)
+            pts = np.ascontiguousarray(np.concatenate(P_), dtype=np.float64)
+            shp = np.ascontiguousarray(np.concatenate(SH), dtype=np.complex128)
+            wa = np.ascontiguousarray(np.concatenate(WA), dtype=np.float64)
+            self.ctx.call("emg_port_probe_set", len(self.ports), ptr(cnt), ptr(own), ptr(pts), ptr(shp), ptr(wa))
+            norms = np.empty(len(self.ports))
+            self.ctx.call("emg_port_norms_get", ptr(norms))
+            self.port_norms = norms
+
     # -------------------------------------------------------------- accessors
     def reduced_pattern(self):
         nnz = ctypes.c_int64()
@@ -207,22 +254,55 @@ class FrequencySweep:
                 coef[row, j] = p.uinc_factor(k)
         return np.ascontiguousarray(k2), np.ascontiguousarray(gam[:, :nb] if nb else gam), np.ascontiguousarray(coef)
 
+    def s_matrix(self, overlaps: np.ndarray, freqs: np.ndarray, caps: bool = True) -> np.ndarray:
+        """S from the device overlaps (mode-overlap ports): for receiving port
+        q and active port j, field_p = (conj(q_m) O - [q==j] |q_m|^2 N_q) / 2Z,
+        mode_p = |q_m|^2 N_q / 2Z, S_qj = field_p / mode_p / sqrt(P_j)
+        (MW/microwave_3d.py:1193-1211, 1294-1303; MW/sparam.py:71-137)."""
+        npt = len(self.ports)
+        S = np.zeros((freqs.shape[0], npt, npt), dtype=np.complex128)
+        for i, f in enumerate(freqs):
+            k0 = 2 * np.pi * f / C0
+            for j, act in enumerate(self.ports):
+                for q, prb in enumerate(self.ports):
+                    qm = prb.mode_factor(k0)
+                    Zm = prb.Zmode(k0)
+                    O = overlaps[i * npt + j, q]
+                    N = self.port_norms[q]
+                    fp = (np.conj(qm) * O - (1.0 if q == j else 0.0) * abs(qm) ** 2 * N) / (2 * Zm)
+                    mp = abs(qm) ** 2 * N / (2 * Zm)
+                    S[i, q, j] = (fp / mp) / np.sqrt(act.power)
+            if caps:
+                S[i] = apply_s_caps(S[i])
+        return S
+
     def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock: int = 16, restart: int = 100,
-              max_cycles: int = 200) -> SweepResult:
+              max_cycles: int = 200, keep_fields: bool = True, method: str = "gmres", idr_s: int = 4,
+              inner_rel32: float = 1e-5) -> SweepResult:
+        """Solve every (frequency, active port) system.  method 'gmres' =
+        GMRES(restart); 'idr' = IDR(idr_s) with a budget of `restart`
+        matvecs per FP64 refinement cycle."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+        self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1}[method], int(idr_s), float(inner_rel32))
         k2, gam, coef = self.batch_coefficients(freqs)
         nbatch = k2.shape[0]
-        x = np.empty((nbatch, self.n_red), dtype=np.complex128)
+        x = np.empty((nbatch, self.n_red), dtype=np.complex128) if keep_fields else None
         relres = np.empty(nbatch)
         iters = np.empty(nbatch, dtype=np.int64)
+        npt = len(self.ports)
+        ov = np.empty((nbatch, npt), dtype=np.complex128) if self.port_norms is not None else None
         prec = {"fp64": 0, "mixed": 1}[precision]
         self.ctx.call("emg_sweep_solve", nbatch, ptr(k2), ptr(gam) if gam.size else None, ptr(coef), prec,
-                      int(fblock), float(tol), int(restart), int(max_cycles), ptr(x), ptr(relres), ptr(iters))
-        npt = len(self.ports)
+                      int(fblock), float(tol), int(restart), int(max_cycles),
+                      ptr(x) if x is not None else None, ptr(relres), ptr(iters),
+                      ptr(ov) if ov is not None else None)
         fields = {}
-        for i in range(freqs.shape[0]):
-            for j, p in enumerate(self.ports):
-                full = np.zeros(self.basis.n_field, dtype=np.complex128)
-                full[self.solve_ids] = x[i * npt + j]
-                fields[(i, p.port_number)] = full
-        return SweepResult(freqs, self.ports, fields, relres.reshape(-1, npt), iters.reshape(-1, npt), precision)
+        if x is not None:
+            for i in range(freqs.shape[0]):
+                for j, p in enumerate(self.ports):
+                    full = np.zeros(self.basis.n_field, dtype=np.complex128)
+                    full[self.solve_ids] = x[i * npt + j]
+                    fields[(i, p.port_number)] = full
+        S = self.s_matrix(ov, freqs) if ov is not None else None
+        return SweepResult(freqs, self.ports, fields, relres.reshape(-1, npt), iters.reshape(-1, npt), precision,
+                           S=S)

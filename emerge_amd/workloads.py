@@ -52,3 +52,19 @@ def waveguide(cells=(11, 5, 18), length: float = 40e-3, jitter: float = 0.0, see
     return VolumeProblem(mesh, basis, np.ones(T, dtype=np.complex128), np.ones(T, dtype=np.complex128),
                          meta={"workload": "wr90_waveguide", "cells": cells, "length": length,
                                "a": WR90_A, "b": WR90_B})
+
+
+def waveguide_bcs(problem: VolumeProblem) -> list:
+    """PEC walls and TE10 RectangularWaveguide ports on z = 0 and z = L
+    (local frame origin at the face centre, x along the broad side)."""
+    from . import bc
+    m = problem.mesh
+    L = problem.meta["length"]
+    a, b = problem.meta["a"], problem.meta["b"]
+    t1 = m.plane_tris(2, 0.0)
+    t2 = m.plane_tris(2, L)
+    pec = np.setdiff1d(m.boundary_tris(), np.concatenate([t1, t2]))
+    cs1 = bc.CoordinateSystem([1, 0, 0], [0, 1, 0], [0, 0, 1], [a / 2, b / 2, 0.0])
+    cs2 = bc.CoordinateSystem([1, 0, 0], [0, 1, 0], [0, 0, 1], [a / 2, b / 2, L])
+    return [bc.PEC(pec), bc.RectangularWaveguide(t1, 1, cs1, (a, b)),
+            bc.RectangularWaveguide(t2, 2, cs2, (a, b))]

@@ -151,7 +151,36 @@ int emg_forcing_get(emg_ctx* ctx, double* out);
  * [nbatch][n_red]; relres_out = final FP64 ||b - A x|| / ||b||. */
 int emg_sweep_solve(emg_ctx* ctx, int64_t nbatch, const double* k2, const double* gamma,
                     const double* rhs_coef, int precision, int fblock, double tol, int restart,
-                    int max_cycles, double* x_out, double* relres_out, int64_t* iters_out);
+                    int max_cycles, double* x_out, double* relres_out, int64_t* iters_out,
+                    double* overlap_out);
+
+/* --------------------------------------------------------- S-parameters */
+
+/* Register the port faces used for S-parameter overlaps: per probe triangle
+ * (ports concatenated, tri_counts[p] each) the tet that owns it, the 6
+ * Dunavant points of gaus_quad_tri(4) ([n][6][3]), the port's mode shape at
+ * those points (complex [n][6][3], without the k0-dependent factor) and the
+ * weight x area products ([n][6]).  With overlap_out non-NULL,
+ * emg_sweep_solve then writes per system and port
+ *   O = sum_tri sum_q w A (E(p_q) . conj(shape(p_q)))  (complex [nbatch][n_ports])
+ * with E interpolated as ned2_tet_interp (E/elements/ned2_interp.py:141-301);
+ * the S-matrix follows from O and the norms below as in _compute_s_data /
+ * sparam_field_power / sparam_mode_power (MW/microwave_3d.py:1250-1303,
+ * MW/sparam.py:71-137). */
+int emg_port_probe_set(emg_ctx* ctx, int32_t n_ports, const int64_t* tri_counts,
+                       const int32_t* owner_tet, const double* pts, const double* shape,
+                       const double* wa);
+/* k0-independent mode norms N_p = sum w A |shape|^2 (n_ports doubles). */
+int emg_port_norms_get(emg_ctx* ctx, double* norms);
+
+/* Krylov method for emg_sweep_solve / emg_csr_solve: 0 = GMRES(restart)
+ * with CGS2 (default), 1 = IDR(idr_s) with bi-orthogonalisation (then
+ * `restart` is the matvec budget of one refinement cycle); inner_rel32 = the
+ * residual reduction an FP32 inner solve aims for before the FP64 residual is
+ * recomputed.  The reference's iterative solvers (scipy gmres/bicgstab/
+ * gcrotmk, E/solver.py:458-527) have no counterpart knob; this selects the
+ * device algorithm. */
+int emg_solver_config(emg_ctx* ctx, int method, int idr_s, double inner_rel32);
 
 /* Generic complex CSR system for the Solver plug-in (replaces
  * Solver.solve(A, b, ...), E/solver.py:365-367, next to SuperLU / PARDISO /
