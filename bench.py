@@ -57,7 +57,8 @@ def cpu_baseline(n_tets_sample: int = 100_000) -> dict:
                       f"scipy COO->CSR, {dt:.1f} s"}
 
 
-def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int) -> dict:
+def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method: str = "gmres",
+                idr_s: int = 4, restart: int = 100) -> dict:
     """WR-90 waveguide (Kuhn 11x5x18, 5,940 tets, 41,118 DOFs), TE10 ports,
     frequency points from a 201-point 8-12 GHz grid, both ports excited per
     point.  Plain FP64 GPU path: FP64 GMRES one (frequency, port) system at a
@@ -71,17 +72,28 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int) -> dic
     grid = np.linspace(8e9, 12e9, 201)
     plain_f = grid[np.linspace(0, 200, n_plain).astype(int)]
     mixed_f = grid[np.linspace(0, 200, n_mixed).astype(int)]
-    sw.solve(plain_f[:1], precision="fp64", fblock=1)  # warm-up (code objects, allocations)
+    kw = dict(method=method, idr_s=idr_s, restart=restart, max_cycles=400 if method == "gmres" else 40,
+              keep_fields=False, inner_rel32=1e-5 if method == "gmres" else 1e-4)
+    kw_m = dict(kw)
+    if method == "idr":  # IDR(s) runs in FP64 only; the mixed path stays GMRES-IR
+        kw_m.update(method="gmres", restart=100, max_cycles=400, inner_rel32=1e-5)
+    print(f"[bench] sweep: warm-up", file=sys.stderr, flush=True)
+    sw.solve(plain_f[:1], precision="fp64", fblock=1, **kw)  # warm-up (code objects, allocations)
+    print(f"[bench] sweep: fp64 plain, {len(plain_f)} points", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    rp = sw.solve(plain_f, precision="fp64", fblock=1)
+    rp = sw.solve(plain_f, precision="fp64", fblock=1, **kw)
     tp = time.perf_counter() - t0
+    print(f"[bench] sweep: fp64 plain {tp:.2f} s; mixed batched, {len(mixed_f)} points", file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
-    rm = sw.solve(mixed_f, precision="mixed", fblock=fblock)
+    rm = sw.solve(mixed_f, precision="mixed", fblock=fblock, **kw_m)
     tm = time.perf_counter() - t0
+    print(f"[bench] sweep: mixed {tm:.2f} s", file=sys.stderr, flush=True)
     ok_p = int(np.sum(np.all(rp.relres <= 1e-8, axis=1)))
     ok_m = int(np.sum(np.all(rm.relres <= 1e-8, axis=1)))
     out = {
         "workload": "WR-90 22.86x10.16x40 mm, Kuhn 11x5x18 (5,940 tets, 41,118 DOFs), TE10 ports, 8-12 GHz",
+        "krylov": f"IDR({idr_s})" if method == "idr" else f"GMRES({restart})",
         "fp64_plain": {"points": len(plain_f), "converged": ok_p, "seconds": round(tp, 3),
                        "freq_points_per_s": round(ok_p / tp, 4),
                        "mean_iters_per_system": float(rp.iters.mean()), "max_relres": float(rp.relres.max())},
@@ -127,6 +139,9 @@ def main() -> int:
     ap.add_argument("--fblock", type=int, default=16)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-only", action="store_true", help="profile helper: run only the solve sweep")
+    ap.add_argument("--method", default="gmres", choices=["gmres", "idr"])
+    ap.add_argument("--idr-s", type=int, default=4)
+    ap.add_argument("--restart", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU sharding is "
@@ -138,7 +153,8 @@ def main() -> int:
     from emerge_amd.workloads import cube_assembly
 
     if args.sweep_only:
-        print(json.dumps({"sweep": sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock, 0)}), flush=True)
+        print(json.dumps({"sweep": sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock, 0, args.method,
+                                               args.idr_s, args.restart)}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -183,7 +199,7 @@ def main() -> int:
     sweep = None
     if not args.no_sweep:
         sweep = sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock,
-                            0 if args.no_cpu_baseline else 2)
+                            0 if args.no_cpu_baseline else 2, args.method, args.idr_s, args.restart)
         print(f"[bench] sweep {json.dumps(sweep)}", file=sys.stderr, flush=True)
 
     line = {

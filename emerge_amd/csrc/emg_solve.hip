@@ -866,7 +866,12 @@ int run_idr(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k
 template <typename R, int F>
 int run_method(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k2, const double2* gam, int nbc,
                const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
-    if (P.method == 1) return run_idr<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
+    if (P.method == 1) {
+        // FP32 IDR(s) recurrences lose bi-orthogonality over the thousands of
+        // steps these systems need and break down; IDR runs in FP64 only.
+        if (sizeof(R) != sizeof(double)) return fail(c, EMG_ERR_ARG, "IDR(s) requires FP64 (precision 0)");
+        return run_idr<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
+    }
     return run_batch<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
 }
 
@@ -905,7 +910,7 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     if (c->n_forcing < 1) return fail(c, EMG_ERR_STATE, "emg_sweep_solve: no forcing vectors");
     if (nbatch < 1 || !k2 || !rhs_coef || (c->n_bc > 0 && !gamma))
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: bad arguments");
-    if (restart < 1 || restart > 1000 || max_cycles < 1 || !(tol > 0))
+    if (restart < 1 || (c->krylov_method == 0 && restart > 1000) || restart > 1000000 || max_cycles < 1 || !(tol > 0))
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: bad solver parameters");
     if (precision != 0 && precision != 1) return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision must be 0 or 1");
     fblock = std::max(1, std::min(fblock, 32));
@@ -1030,7 +1035,7 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
     EMG_CHECK_CTX(c);
     if (c->g_n == 0) return fail(c, EMG_ERR_STATE, "emg_csr_solve: no matrix (call emg_csr_set)");
     if (nrhs < 1 || !b || !x) return fail(c, EMG_ERR_ARG, "emg_csr_solve: bad arguments");
-    if (restart < 1 || restart > 1000 || max_cycles < 1 || !(tol > 0))
+    if (restart < 1 || (c->krylov_method == 0 && restart > 1000) || restart > 1000000 || max_cycles < 1 || !(tol > 0))
         return fail(c, EMG_ERR_ARG, "emg_csr_solve: bad solver parameters");
     if (precision != 0 && precision != 1) return fail(c, EMG_ERR_ARG, "emg_csr_solve: precision must be 0 or 1");
     hipStream_t s = c->stream;

@@ -135,11 +135,17 @@ def test_solver_plugin_matches_superlu():
     assert rep3.aux["precision"] == "mixed"
 
 
-@pytest.mark.parametrize("precision,s", [("fp64", 4), ("mixed", 4), ("mixed", 8)])
+def test_idr_rejects_fp32(sweep):
+    from emerge_amd._lib import EmgError
+    with pytest.raises(EmgError):
+        sweep.solve([10e9], precision="mixed", method="idr", restart=100, max_cycles=1)
+
+
+@pytest.mark.parametrize("precision,s", [("fp64", 4), ("fp64", 8)])
 def test_sweep_idr_matches_superlu(sweep, precision, s):
     print(f"idr precision={precision} s={s}", flush=True)
     freqs = np.array([8e9, 10e9])
-    res = sweep.solve(freqs, precision=precision, tol=1e-8, fblock=4, restart=2000, max_cycles=50,
+    res = sweep.solve(freqs, precision=precision, tol=1e-8, fblock=4, restart=30000, max_cycles=8,
                       method="idr", idr_s=s, inner_rel32=1e-4)
     assert np.all(res.relres <= 1e-8), res.relres
     for i, f in enumerate(freqs):
