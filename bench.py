@@ -142,11 +142,26 @@ def main() -> int:
     ap.add_argument("--method", default="gmres", choices=["gmres", "idr"])
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="counter calibration: only run 3 device-to-device copies of this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU sharding is "
               "documented, not launched", file=sys.stderr)
         return 2
+
+    if args.copy_probe_mib > 0:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings
+        # calibrate the counters used for the assembly kernel's HBM traffic.
+        import torch
+        nb = args.copy_probe_mib << 20
+        a = torch.ones(nb, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb}), flush=True)
+        return 0
 
     from emerge_amd._lib import Context
     from emerge_amd.assembler import MeshOnDevice

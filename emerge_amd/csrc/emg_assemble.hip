@@ -26,6 +26,8 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
     double2* __restrict__ Bv) {
     __shared__ double stage[20][kAsmThreads];  // element row of every lane
     __shared__ double fac[2][kAsmThreads];     // complex row factor of every lane
+    __shared__ uint2 s_code4[5 * kAsmThreads];  // contributor codes of the chunk (u16, 4 per uint2)
+    __shared__ uint16_t s_beg[kAsmMaxSlots + 1];  // first contributor of every slot, + sentinel
 
     // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so
     // give each XCD a contiguous range of chunks (neighbouring rows share
@@ -37,6 +39,19 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
     const Chunk ch = chunks[ci];
     const int tid = threadIdx.x;
     const bool active = tid < ch.ninc;
+    const int ncontrib = 20 * ch.ninc;
+
+    // Stage the chunk's schedule in LDS with coalesced loads, issued before
+    // the geometry gathers so both are in flight together; the two slot
+    // reductions below then touch only LDS.  The contributor list starts at
+    // 40*inc0 bytes (8-byte aligned) and holds a multiple of 4 codes.
+    {
+        const uint2* cb4 = reinterpret_cast<const uint2*>(contrib + int64_t(ch.inc0) * 20);
+        for (int k = tid; k < ncontrib / 4; k += kAsmThreads) s_code4[k] = cb4[k];
+        for (int k = tid; k < ch.nslot; k += kAsmThreads) s_beg[k] = slot_beg[int64_t(ch.slot0) + k];
+        if (tid == 0) s_beg[ch.nslot] = uint16_t(ncontrib);
+    }
+    const uint16_t* s_code = reinterpret_cast<const uint16_t*>(s_code4);
 
     int u = 0;
     ned2::TetGeom G;
@@ -71,8 +86,6 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
         m_im = ep.y * cm;
     }
 
-    const uint16_t* cb = contrib + int64_t(ch.inc0) * 20;
-    const int ncontrib = 20 * ch.ninc;
     // pass 0: curl-curl -> Ev; pass 1: mass -> Bv
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
@@ -120,18 +133,17 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
         __syncthreads();
         double2* out = pass == 0 ? Ev : Bv;
         for (int k = tid; k < ch.nslot; k += kAsmThreads) {
-            const int64_t s = int64_t(ch.slot0) + k;
-            const int j0 = slot_beg[s];
-            const int j1 = (k + 1 < ch.nslot) ? int(slot_beg[s + 1]) : ncontrib;
+            const int j0 = s_beg[k];
+            const int j1 = s_beg[k + 1];
             double re = 0.0, im = 0.0;
             for (int j = j0; j < j1; ++j) {
-                const int code = cb[j];
+                const int code = s_code[j];
                 const int ln = code >> 5;
                 const double x = stage[code & 31][ln];
                 re = __builtin_fma(fac[0][ln], x, re);
                 im = __builtin_fma(fac[1][ln], x, im);
             }
-            out[s] = make_double2(re, im);
+            out[int64_t(ch.slot0) + k] = make_double2(re, im);
         }
         __syncthreads();
     }
