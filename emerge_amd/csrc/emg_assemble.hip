@@ -5,12 +5,13 @@
 // duplicate sum (ASM/curlcurl.py:145-173, 180-412, 419-466).  Each workgroup
 // owns a chunk of consecutive CSR rows.  Every lane takes one incidence (tet,
 // canonical row u) of those rows, computes the tet's canonical geometry and
-// row u of both element matrices, and adds it into an LDS copy of the chunk's
-// CSR slots through the precomputed scatter offsets.  Lanes add in phases of
-// their rank within the row (= tet order), so every CSR value is summed in a
-// fixed order: results are bitwise reproducible and no element matrix or COO
-// list ever goes to HBM.  HBM traffic per tet: ~1 KB of schedule + the
-// written CSR values + cached vertex/material gathers.
+// row u of the element matrices into LDS; then every CSR slot of the chunk is
+// summed over its precomputed contributor list (lane, column), which is in
+// increasing tet order -- the order in which scipy's COO->CSR duplicate sum
+// adds the reference's COO entries.  Results are bitwise reproducible (no
+// atomics) and no element matrix or COO list ever goes to HBM.  HBM traffic
+// per tet: ~1.5 KB of schedule, the vertex/material gathers (mostly L2 hits)
+// and the written CSR values.
 #include "emg_common.h"
 #include "emg_ned2.h"
 
@@ -18,16 +19,22 @@ namespace {
 
 using namespace emg;
 
+// One workgroup per chunk; two passes (curl-curl, then mass).  Slot sums
+// are segmented over the contributor list: lane l takes codes
+// [20 l, 20 l + 20), finishes the slots that end in its range and passes the
+// partial sum of a slot continuing from lane l-1 to that slot's first lane
+// (LDS carry), which adds the carries in lane order.  Every lane does the
+// same work whatever the slot sizes.
 __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
     const Chunk* __restrict__ chunks, int64_t n_chunks, const int32_t* __restrict__ inc_tet,
-    const uint8_t* __restrict__ inc_u, const uint16_t* __restrict__ slot_beg, const uint16_t* __restrict__ contrib,
-    const double* __restrict__ xyz, int64_t V, const int32_t* __restrict__ tets, int64_t T,
+    const uint8_t* __restrict__ inc_u, const int4* __restrict__ inc_vid, const uint16_t* __restrict__ grp_slot,
+    const uint16_t* __restrict__ contrib, const double* __restrict__ xyz, int64_t V,
     const double2* __restrict__ eps, const double2* __restrict__ inu, double2* __restrict__ Ev,
     double2* __restrict__ Bv) {
     __shared__ double stage[20][kAsmThreads];  // element row of every lane
-    __shared__ double fac[2][kAsmThreads];     // complex row factor of every lane
-    __shared__ uint2 s_code4[5 * kAsmThreads];  // contributor codes of the chunk (u16, 4 per uint2)
-    __shared__ uint16_t s_beg[kAsmMaxSlots + 1];  // first contributor of every slot, + sentinel
+    __shared__ double2 fac[kAsmThreads];       // complex row factor of every lane
+    __shared__ double2 carry[kAsmThreads];     // partial sum of a slot continued from the previous lane
+    __shared__ int closed[kAsmThreads];        // whether that slot ends in this lane's range
 
     // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so
     // give each XCD a contiguous range of chunks (neighbouring rows share
@@ -39,30 +46,30 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
     const Chunk ch = chunks[ci];
     const int tid = threadIdx.x;
     const bool active = tid < ch.ninc;
-    const int ncontrib = 20 * ch.ninc;
-
-    // Stage the chunk's schedule in LDS with coalesced loads, issued before
-    // the geometry gathers so both are in flight together; the two slot
-    // reductions below then touch only LDS.  The contributor list starts at
-    // 40*inc0 bytes (8-byte aligned) and holds a multiple of 4 codes.
-    {
-        const uint2* cb4 = reinterpret_cast<const uint2*>(contrib + int64_t(ch.inc0) * 20);
-        for (int k = tid; k < ncontrib / 4; k += kAsmThreads) s_code4[k] = cb4[k];
-        for (int k = tid; k < ch.nslot; k += kAsmThreads) s_beg[k] = slot_beg[int64_t(ch.slot0) + k];
-        if (tid == 0) s_beg[ch.nslot] = uint16_t(ncontrib);
-    }
-    const uint16_t* s_code = reinterpret_cast<const uint16_t*>(s_code4);
 
     int u = 0;
     ned2::TetGeom G;
-    double e_re = 0, e_im = 0, m_re = 0, m_im = 0;
+    double2 kf = make_double2(0, 0), mf = make_double2(0, 0);
+    uint32_t cw[10];  // this lane's 20 contributor codes, two per word
+    int slot_first = 0;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) cw[r] = 0;
     if (active) {
         const int64_t inc = int64_t(ch.inc0) + tid;
+        // 20 codes = 40 bytes at 40*inc: 8-byte aligned
+        const uint2* cp = reinterpret_cast<const uint2*>(contrib + inc * 20);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const uint2 v = cp[r];
+            cw[2 * r] = v.x;
+            cw[2 * r + 1] = v.y;
+        }
+        slot_first = grp_slot[inc];
         const int t = inc_tet[inc];
         u = inc_u[inc];
-        int w[4], rk[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) w[k] = tets[k * T + t];
+        const int4 wv = inc_vid[inc];
+        const int w[4] = {wv.x, wv.y, wv.z, wv.w};
+        int rk[4];
         ned2::ranks4(w, rk);
         double X[4][3];
 #pragma unroll
@@ -71,20 +78,20 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
             X[k][1] = xyz[V + w[k]];
             X[k][2] = xyz[2 * V + w[k]];
         }
+        const double2 mu = inu[t];
+        const double2 ep = eps[t];
         ned2::tet_geometry(X, rk, G);
         double su = 0.0;
 #pragma unroll
         for (int k = 0; k < 20; ++k) su = (k == u) ? G.s[k] : su;
         const double i3 = G.inv6v * G.inv6v * G.inv6v;
-        const double2 mu = inu[t];
-        const double2 ep = eps[t];
         const double ck = su * i3 * (1.0 / 120.0);
         const double cm = su * G.inv6v * (1.0 / 5040.0);
-        e_re = mu.x * ck;
-        e_im = mu.y * ck;
-        m_re = ep.x * cm;
-        m_im = ep.y * cm;
+        kf = make_double2(mu.x * ck, mu.y * ck);
+        mf = make_double2(ep.x * cm, ep.y * cm);
     }
+    const bool cont = (cw[0] & kCodeCont) != 0;
+    const bool tail_open = active && (cw[9] & (uint32_t(kCodeLast) << 16)) == 0;
 
     // pass 0: curl-curl -> Ev; pass 1: mass -> Bv
 #pragma unroll 1
@@ -99,53 +106,71 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
                 const int u0 = __builtin_amdgcn_readfirstlane(__shfl(u, leader));
                 if (pending && u == u0) {
                     // The row arithmetic is loop invariant and speculatable;
-                    // opaque copies of the inputs stop the compiler from
-                    // hoisting all 20 switch cases out of this loop.
-                    double sv[20];
+                    // marking its inputs as rewritten on every trip (an empty
+                    // asm with in/out operands: no instruction) stops the
+                    // compiler from hoisting all 20 switch cases out of this
+                    // loop.
 #pragma unroll
-                    for (int k = 0; k < 20; ++k) {
-                        sv[k] = G.s[k];
-                        asm volatile("" : "+v"(sv[k]));
-                    }
+                    for (int k = 0; k < 20; ++k) asm volatile("" : "+v"(G.s[k]));
                     if (pass == 0) {
-                        double hv[21];
 #pragma unroll
-                        for (int k = 0; k < 21; ++k) {
-                            hv[k] = G.h[k];
-                            asm volatile("" : "+v"(hv[k]));
-                        }
-                        ned2::stiff_row(u0, hv, sv, &stage[0][tid], kAsmThreads);
+                        for (int k = 0; k < 21; ++k) asm volatile("" : "+v"(G.h[k]));
+                        ned2::stiff_row(u0, G.h, G.s, &stage[0][tid], kAsmThreads);
                     } else {
-                        double gv[10];
 #pragma unroll
-                        for (int k = 0; k < 10; ++k) {
-                            gv[k] = G.g[k];
-                            asm volatile("" : "+v"(gv[k]));
-                        }
-                        ned2::mass_row(u0, gv, sv, &stage[0][tid], kAsmThreads);
+                        for (int k = 0; k < 10; ++k) asm volatile("" : "+v"(G.g[k]));
+                        ned2::mass_row(u0, G.g, G.s, &stage[0][tid], kAsmThreads);
                     }
                     pending = false;
                 }
             }
-            fac[0][tid] = pass == 0 ? e_re : m_re;
-            fac[1][tid] = pass == 0 ? e_im : m_im;
+            fac[tid] = make_double2(pass == 0 ? kf.x : mf.x, pass == 0 ? kf.y : mf.y);
         }
         __syncthreads();
-        double2* out = pass == 0 ? Ev : Bv;
-        for (int k = tid; k < ch.nslot; k += kAsmThreads) {
-            const int j0 = s_beg[k];
-            const int j1 = s_beg[k + 1];
-            double re = 0.0, im = 0.0;
-            for (int j = j0; j < j1; ++j) {
-                const int code = s_code[j];
-                const int ln = code >> 5;
+        double2* out = (pass == 0 ? Ev : Bv) + ch.slot0;
+        int slot = slot_first;
+        bool first = true;
+        double re = 0.0, im = 0.0;
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < 20; ++r) {
+                const uint32_t code = (cw[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                const int ln = (code >> 5) & 0xff;
                 const double x = stage[code & 31][ln];
-                re = __builtin_fma(fac[0][ln], x, re);
-                im = __builtin_fma(fac[1][ln], x, im);
+                const double2 f = fac[ln];
+                re = __builtin_fma(f.x, x, re);
+                im = __builtin_fma(f.y, x, im);
+                if (code & kCodeLast) {
+                    if (first && cont) {
+                        carry[tid] = make_double2(re, im);
+                        closed[tid] = 1;
+                    } else {
+                        out[slot] = make_double2(re, im);
+                    }
+                    ++slot;
+                    re = 0.0;
+                    im = 0.0;
+                    first = false;
+                }
             }
-            out[int64_t(ch.slot0) + k] = make_double2(re, im);
+            if (tail_open && first && cont) {  // the whole range lies inside one slot
+                carry[tid] = make_double2(re, im);
+                closed[tid] = 0;
+            }
         }
         __syncthreads();
+        // a slot that starts in this lane's range and runs past it: add the
+        // following lanes' partial sums in order (the chunk's last code
+        // always ends a slot, so the walk stops inside the chunk)
+        if (tail_open && !(first && cont)) {
+            for (int q = tid + 1; q < ch.ninc; ++q) {
+                const double2 cq = carry[q];
+                re += cq.x;
+                im += cq.y;
+                if (closed[q]) break;
+            }
+            out[slot] = make_double2(re, im);
+        }
     }
 }
 
@@ -159,9 +184,9 @@ extern "C" int emg_assemble(emg_ctx* c) {
     EMG_HIP(c, c->Bv.alloc(2 * c->nnz));
     const int64_t per = (c->n_chunks + 7) / 8;
     k_assemble_rows<<<unsigned(per * 8), kAsmThreads, 0, c->stream>>>(
-        c->chunks.p, c->n_chunks, c->inc_tet.p, c->inc_u.p, c->slot_beg.p, c->contrib.p, c->xyz.p, c->n_nodes,
-        c->tets.p,
-        c->n_tets, reinterpret_cast<const double2*>(c->eps.p), reinterpret_cast<const double2*>(c->inu.p),
+        c->chunks.p, c->n_chunks, c->inc_tet.p, c->inc_u.p, reinterpret_cast<const int4*>(c->inc_vid.p),
+        c->grp_slot.p, c->contrib.p, c->xyz.p, c->n_nodes, reinterpret_cast<const double2*>(c->eps.p),
+        reinterpret_cast<const double2*>(c->inu.p),
         reinterpret_cast<double2*>(c->Ev.p), reinterpret_cast<double2*>(c->Bv.p));
     EMG_HIP(c, hipGetLastError());
     c->have_values = true;

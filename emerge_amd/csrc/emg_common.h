@@ -53,6 +53,8 @@ struct Chunk {
 };
 
 constexpr int kAsmThreads = 256;
+constexpr uint16_t kCodeLast = 0x8000;  // contributor code flag: last of its CSR slot
+constexpr uint16_t kCodeCont = 0x2000;  // flag: group start that continues the previous group's slot
 constexpr int kAsmMaxSlots = 4096;
 
 }  // namespace emg
@@ -80,8 +82,9 @@ struct emg_ctx {
     int64_t n_chunks = 0, n_inc = 0, max_rank = 0;
     emg::DBuf<int32_t> inc_tet;   // tet id per incidence (chunk-local order: by canonical row)
     emg::DBuf<uint8_t> inc_u;     // canonical element row of the incidence
-    emg::DBuf<uint16_t> slot_beg; // per CSR slot: first chunk-relative contributor
-    emg::DBuf<uint16_t> contrib;  // 20 per incidence: (lane << 5) | canonical column, grouped by slot
+    emg::DBuf<int32_t> inc_vid;   // the incidence's tet vertex ids, 4 per incidence (16-byte records)
+    emg::DBuf<uint16_t> grp_slot; // per group of 20 contributor codes: chunk-relative slot of its first
+    emg::DBuf<uint16_t> contrib;  // 20 per incidence: (lane << 5) | canonical column | flags, grouped by slot
 
     // materials and values
     bool have_material = false;

@@ -93,21 +93,26 @@ __device__ __forceinline__ void canonical_map(const int* w, int* umap) {
     }
 }
 
-// One workgroup per chunk.  Per incidence: its tet and canonical row u
-// (incidences stably reordered by u inside the chunk, so that a wavefront
-// evaluates few distinct element rows).  Per CSR slot of the chunk: the list
-// of (incidence lane, canonical column) contributors, in rank order within
-// the row (= increasing tet id), which fixes the summation order.
+// One workgroup per chunk.  Per incidence: its tet, vertex ids and canonical
+// row u (incidences stably reordered by u inside the chunk, so that a
+// wavefront evaluates few distinct element rows).  The chunk's 20*ninc
+// contributor codes (incidence lane << 5 | canonical column) are grouped by
+// CSR slot, each slot's list in rank order within the row (= increasing tet
+// id), which fixes the summation order.  Flags: kCodeLast on the last code of
+// every slot, kCodeCont on a code at a group start (index % 20 == 0) that
+// continues the previous group's slot.  Per group of 20 codes: the
+// chunk-relative slot of its first code.
 __global__ __launch_bounds__(kAsmThreads) void k_schedule(
     const Chunk* __restrict__ chunks, const uint32_t* __restrict__ inc_code, const uint32_t* __restrict__ inc_row,
     const int32_t* __restrict__ incptr, const int32_t* __restrict__ tets, const int32_t* __restrict__ cmap, int64_t T,
-    int32_t* __restrict__ out_tet, uint8_t* __restrict__ out_u, uint16_t* __restrict__ slot_beg,
-    uint16_t* __restrict__ contrib, int32_t* __restrict__ err) {
+    int32_t* __restrict__ out_tet, uint8_t* __restrict__ out_u, int4* __restrict__ out_vid,
+    uint16_t* __restrict__ grp_slot, uint16_t* __restrict__ contrib, int32_t* __restrict__ err) {
     __shared__ int su[kAsmThreads];
     __shared__ int cnt[kAsmMaxSlots + 1];
     __shared__ int fillc[kAsmMaxSlots];
     __shared__ uint16_t tcode[20 * kAsmThreads];
     __shared__ uint8_t trank[20 * kAsmThreads];
+    __shared__ uint16_t gslot[kAsmThreads];
     const Chunk ch = chunks[blockIdx.x];
     const int l = threadIdx.x;
     for (int k = l; k <= ch.nslot; k += kAsmThreads) cnt[k] = 0;
@@ -115,6 +120,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_schedule(
     int u = 1 << 20;
     int t = 0, rank = 0;
     int off[20];
+    int w[4] = {0, 0, 0, 0};
     if (l < ch.ninc) {
         const int64_t inc = int64_t(ch.inc0) + l;
         const uint32_t code = inc_code[inc];
@@ -122,7 +128,6 @@ __global__ __launch_bounds__(kAsmThreads) void k_schedule(
         const int i = int(code % 20);
         const int row = int(inc_row[inc]);
         rank = int(inc - incptr[row]);
-        int w[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) w[k] = tets[k * T + t];
         int umap[20], vmap[20];
@@ -179,14 +184,20 @@ __global__ __launch_bounds__(kAsmThreads) void k_schedule(
             tcode[z + 1] = cc;
             trank[z + 1] = rr;
         }
-        slot_beg[int64_t(ch.slot0) + k] = uint16_t(b);
+        tcode[e - 1] |= kCodeLast;  // every slot has at least one contributor
+        for (int g = (b + 19) / 20; 20 * g < e; ++g) {
+            gslot[g] = uint16_t(k);
+            if (20 * g > b) tcode[20 * g] |= kCodeCont;
+        }
     }
     __syncthreads();
     if (l < ch.ninc) {
         out_tet[int64_t(ch.inc0) + pos] = t;
         out_u[int64_t(ch.inc0) + pos] = uint8_t(u);
+        out_vid[int64_t(ch.inc0) + pos] = make_int4(w[0], w[1], w[2], w[3]);
     }
     for (int j = l; j < 20 * ch.ninc; j += kAsmThreads) contrib[int64_t(ch.inc0) * 20 + j] = tcode[j];
+    if (l < ch.ninc) grp_slot[int64_t(ch.inc0) + l] = gslot[l];
 }
 
 inline unsigned grid_for(int64_t n, int block) { return unsigned((n + block - 1) / block); }
@@ -305,14 +316,16 @@ extern "C" int emg_pattern_build(emg_ctx* c, int64_t* nnz_out) {
     EMG_HIP(c, hipMemcpyAsync(c->chunks.p, hch.data(), sizeof(Chunk) * hch.size(), hipMemcpyHostToDevice, st));
     EMG_HIP(c, c->inc_tet.alloc(MI));
     EMG_HIP(c, c->inc_u.alloc(MI));
-    EMG_HIP(c, c->slot_beg.alloc(nnz));
+    EMG_HIP(c, c->inc_vid.alloc(4 * MI));
+    EMG_HIP(c, c->grp_slot.alloc(MI));
     EMG_HIP(c, c->contrib.alloc(MI * 20));
     DBuf<int32_t> derr;
     EMG_HIP(c, derr.alloc(1));
     EMG_HIP(c, hipMemsetAsync(derr.p, 0, sizeof(int32_t), st));
     k_schedule<<<unsigned(hch.size()), kAsmThreads, 0, st>>>(c->chunks.p, ivout.p, ikout.p, incptr.p, c->tets.p,
                                                              c->cmap.p, T, c->inc_tet.p, c->inc_u.p,
-                                                             c->slot_beg.p, c->contrib.p, derr.p);
+                                                             reinterpret_cast<int4*>(c->inc_vid.p),
+                                                             c->grp_slot.p, c->contrib.p, derr.p);
     EMG_HIP(c, hipGetLastError());
     int32_t herr = 0;
     EMG_HIP(c, hipMemcpyAsync(&herr, derr.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
