@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
         ned2::tet_geometry(X, rk, G);
         double su = 0.0;
 #pragma unroll
-        for (int k = 0; k < 20; ++k) su = (k == u) ? G.s[k] : su;
+        for (int k = 0; k < 20; ++k) su = (k == u) ? G.d[ned2::len_index(k)] : su;
         const double i3 = G.inv6v * G.inv6v * G.inv6v;
         const double ck = su * i3 * (1.0 / 120.0);
         const double cm = su * G.inv6v * (1.0 / 5040.0);
@@ -111,15 +111,15 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
                     // compiler from hoisting all 20 switch cases out of this
                     // loop.
 #pragma unroll
-                    for (int k = 0; k < 20; ++k) asm volatile("" : "+v"(G.s[k]));
+                    for (int k = 0; k < 6; ++k) asm volatile("" : "+v"(G.d[k]));
                     if (pass == 0) {
 #pragma unroll
                         for (int k = 0; k < 21; ++k) asm volatile("" : "+v"(G.h[k]));
-                        ned2::stiff_row(u0, G.h, G.s, &stage[0][tid], kAsmThreads);
+                        ned2::stiff_row(u0, G.h, G.d, &stage[0][tid], kAsmThreads);
                     } else {
 #pragma unroll
                         for (int k = 0; k < 10; ++k) asm volatile("" : "+v"(G.g[k]));
-                        ned2::mass_row(u0, G.g, G.s, &stage[0][tid], kAsmThreads);
+                        ned2::mass_row(u0, G.g, G.d, &stage[0][tid], kAsmThreads);
                     }
                     pending = false;
                 }

@@ -158,18 +158,22 @@ __device__ __forceinline__ double contract_m(const double* g, std::integer_seque
     return acc;
 }
 
+// Length-factor index of DOF u: s_u = d[len_index(u)] with d the 6 edge
+// lengths of the canonical tet.
+__host__ __device__ constexpr int len_index(int u) { return edge_index(dof(u).la, dof(u).lb); }
+
 // Row U of the unscaled element matrices: kr[v] = s_v * (TK h)[U][v] and
 // mr[v] = s_v * (TM g)[U][v] (computed in separate passes to bound the
-// number of live registers).
+// number of live registers); s_v = d[len_index(v)].
 template <int U, int... V>
-__device__ __forceinline__ void stiff_row_impl(const double* h, const double* s, double* kr, int stride,
+__device__ __forceinline__ void stiff_row_impl(const double* h, const double* d, double* kr, int stride,
                                                std::integer_sequence<int, V...>) {
-    ((kr[V * stride] = s[V] * contract_k<U, V>(h, std::make_integer_sequence<int, 21>{})), ...);
+    ((kr[V * stride] = d[len_index(V)] * contract_k<U, V>(h, std::make_integer_sequence<int, 21>{})), ...);
 }
 template <int U, int... V>
-__device__ __forceinline__ void mass_row_impl(const double* g, const double* s, double* mr, int stride,
+__device__ __forceinline__ void mass_row_impl(const double* g, const double* d, double* mr, int stride,
                                               std::integer_sequence<int, V...>) {
-    ((mr[V * stride] = s[V] * contract_m<U, V>(g, std::make_integer_sequence<int, 10>{})), ...);
+    ((mr[V * stride] = d[len_index(V)] * contract_m<U, V>(g, std::make_integer_sequence<int, 10>{})), ...);
 }
 
 // Dispatch on a wave-uniform canonical row index.
@@ -185,13 +189,13 @@ __device__ __forceinline__ void mass_row_impl(const double* g, const double* s, 
     }
 
 // out[v * stride] receives column v of row u.
-__device__ __forceinline__ void stiff_row(int u, const double* h, const double* s, double* kr, int stride) {
-#define EMG_K_CALL(n) stiff_row_impl<n>(h, s, kr, stride, std::make_integer_sequence<int, 20>{})
+__device__ __forceinline__ void stiff_row(int u, const double* h, const double* d, double* kr, int stride) {
+#define EMG_K_CALL(n) stiff_row_impl<n>(h, d, kr, stride, std::make_integer_sequence<int, 20>{})
     EMG_ROW_SWITCH(EMG_K_CALL)
 #undef EMG_K_CALL
 }
-__device__ __forceinline__ void mass_row(int u, const double* g, const double* s, double* mr, int stride) {
-#define EMG_M_CALL(n) mass_row_impl<n>(g, s, mr, stride, std::make_integer_sequence<int, 20>{})
+__device__ __forceinline__ void mass_row(int u, const double* g, const double* d, double* mr, int stride) {
+#define EMG_M_CALL(n) mass_row_impl<n>(g, d, mr, stride, std::make_integer_sequence<int, 20>{})
     EMG_ROW_SWITCH(EMG_M_CALL)
 #undef EMG_M_CALL
 }
@@ -200,7 +204,7 @@ __device__ __forceinline__ void mass_row(int u, const double* g, const double* s
 struct TetGeom {
     double g[10];
     double h[21];
-    double s[20];
+    double d[6];   // edge lengths; DOF u scales with d[len_index(u)]
     double inv6v;
 };
 
@@ -276,10 +280,7 @@ __device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int
         for (int f = e; f < 6; ++f)
             G.h[cross_gram_index(e, f)] = c[e][0] * c[f][0] + c[e][1] * c[f][1] + c[e][2] * c[f][2];
 #pragma unroll
-    for (int u = 0; u < 20; ++u) {
-        const Dof D = dof(u);
-        G.s[u] = d[edge_index(D.la, D.lb)];
-    }
+    for (int e = 0; e < 6; ++e) G.d[e] = d[e];
 }
 
 }  // namespace ned2
