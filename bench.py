@@ -73,7 +73,7 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     plain_f = grid[np.linspace(0, 200, n_plain).astype(int)]
     mixed_f = grid[np.linspace(0, 200, n_mixed).astype(int)]
     kw = dict(method=method, idr_s=idr_s, restart=restart, max_cycles=400 if method == "gmres" else 40,
-              keep_fields=False, inner_rel32=1e-5 if method == "gmres" else 1e-4)
+              keep_fields=False, inner_rel32={"gmres": 1e-5, "idr": 1e-4, "cocg": 1e-2}[method])
     kw_m = dict(kw)
     if method == "idr":  # IDR(s) runs in FP64 only; the mixed path stays GMRES-IR
         kw_m.update(method="gmres", restart=100, max_cycles=400, inner_rel32=1e-5)
@@ -93,7 +93,8 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     ok_m = int(np.sum(np.all(rm.relres <= 1e-8, axis=1)))
     out = {
         "workload": "WR-90 22.86x10.16x40 mm, Kuhn 11x5x18 (5,940 tets, 41,118 DOFs), TE10 ports, 8-12 GHz",
-        "krylov": f"IDR({idr_s})" if method == "idr" else f"GMRES({restart})",
+        "krylov": {"idr": f"IDR({idr_s})", "gmres": f"GMRES({restart})",
+                   "cocg": "COCG on (A+A^T)/2 in FP64 refinement"}[method],
         "fp64_plain": {"points": len(plain_f), "converged": ok_p, "seconds": round(tp, 3),
                        "freq_points_per_s": round(ok_p / tp, 4),
                        "mean_iters_per_system": float(rp.iters.mean()), "max_relres": float(rp.relres.max())},
@@ -139,7 +140,7 @@ def main() -> int:
     ap.add_argument("--fblock", type=int, default=16)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-only", action="store_true", help="profile helper: run only the solve sweep")
-    ap.add_argument("--method", default="gmres", choices=["gmres", "idr"])
+    ap.add_argument("--method", default="gmres", choices=["gmres", "idr", "cocg"])
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--copy-probe-mib", type=int, default=0,

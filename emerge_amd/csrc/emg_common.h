@@ -101,6 +101,9 @@ struct emg_ctx {
     emg::DBuf<int32_t> red_ptr, red_idx, red_map, red_diag;
     emg::DBuf<double> E64, B64;   // 2 x nnz_red
     emg::DBuf<float> E32, B32;    // 2 x nnz_red
+    bool have_bsym = false;       // symmetric part (B + B^T)/2 of the reduced mass values built
+    emg::DBuf<double> Bs64;       // 2 x nnz_red
+    emg::DBuf<float> Bs32;        // 2 x nnz_red
 
     // boundary triangles (all BCs concatenated) and reduced surface terms
     int64_t n_tri_total = 0;
@@ -126,7 +129,7 @@ struct emg_ctx {
     std::vector<double> port_norm;        // sum w a |shape|^2 per port
 
     // Krylov method of the solves (emg_solver_config)
-    int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s)
+    int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part
     int idr_s = 4;
     double inner_rel32 = 1e-5;    // FP32 inner reduction per refinement step
 
@@ -135,6 +138,9 @@ struct emg_ctx {
     emg::DBuf<int32_t> g_ptr, g_idx, g_diag;
     emg::DBuf<double> g_v64;
     emg::DBuf<float> g_v32;
+    bool g_have_sym = false;      // symmetric part (A + A^T)/2 of the generic matrix built
+    emg::DBuf<double> g_s64;
+    emg::DBuf<float> g_s32;
 };
 
 namespace emg {

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "emg_common.h"
@@ -863,6 +864,485 @@ int run_idr(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k
     return EMG_OK;
 }
 
+
+// -------------------------------------------------------------------- COCG
+// Conjugate orthogonal conjugate gradients (van der Vorst & Melissen, IEEE
+// Trans. Magn. 26(2), 1990) with Jacobi preconditioning, on the complex-
+// symmetric part A_s = (A + A^T)/2 of the system, restarted inside FP64
+// iterative refinement on the true A.  The reference's mass matrix is not
+// symmetric (its face-face block uses the monomial of ASM/curlcurl.py:361),
+// so COCG alone would stall; as an inner solver for the refinement
+// r = b - A x, x += A_s^{-1} r it converges, each cycle gaining the factor
+// ||A_s^{-1}(A - A_s)||.  Per iteration: one SpMV (the direction
+// p = z + beta p formed on the fly) writing partial sums of p^T q, one
+// single-workgroup kernel that adds them in a fixed order and sets alpha,
+// one update pass writing partial sums of r^T z and ||r||^2, and one that
+// sets beta and the stop flags.  Scalars stay on the device, blocks of
+// iterations are replayed as one HIP graph, and the host polls the
+// per-system stop flags between blocks.
+
+constexpr int kCgThreads = 1024;    // threads per workgroup of the COCG kernels
+constexpr int kCgMaxBlocks = 2048;  // grid cap (partial sums per kernel)
+
+struct CgState {
+    double2* rho;    // [F] r^T z
+    double2* beta;   // [F]
+    double2* alpha;  // [F]
+    double* rr;      // [F] ||r||^2 of the (scaled) inner residual
+    double* tgt2;    // [F] inner stop: rr <= tgt2
+    int* done;       // [F] inner solve finished (converged or broke down)
+    double2* part;   // [kCgMaxBlocks][2F] partial sums
+};
+
+// slot of (col, row) for every slot of a structurally symmetric CSR pattern
+__global__ void k_tmap(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                       int32_t* __restrict__ tmap, int32_t* __restrict__ err) {
+    const int64_t row = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    for (int j = ptr[row]; j < ptr[row + 1]; ++j) {
+        const int c = idx[j];
+        int lo = ptr[c], hi = ptr[c + 1] - 1, found = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const int v = idx[mid];
+            if (v < row) lo = mid + 1;
+            else if (v > row) hi = mid - 1;
+            else { found = mid; break; }
+        }
+        if (found < 0) {
+            atomicExch(err, 1);
+            found = j;
+        }
+        tmap[j] = found;
+    }
+}
+
+// s = (v + v^T) / 2, FP64 and FP32 copies (exactly symmetric: the sum is
+// formed from the same two operands for both slots)
+__global__ void k_symmetrize(int64_t nnz, const int32_t* __restrict__ tmap, const double2* __restrict__ v,
+                             double2* __restrict__ s64, float2* __restrict__ s32) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const double2 a = v[k], b = v[tmap[k]];
+    const double2 m = make_double2((a.x + b.x) * 0.5, (a.y + b.y) * 0.5);
+    s64[k] = m;
+    s32[k] = make_float2(float(m.x), float(m.y));
+}
+
+int build_symmetric(emg_ctx* c, int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* idx, const double* v,
+                    DBuf<double>& s64, DBuf<float>& s32) {
+    hipStream_t s = c->stream;
+    DBuf<int32_t> tmap, err;
+    EMG_HIP(c, tmap.alloc(std::max<int64_t>(nnz, 1)));
+    EMG_HIP(c, err.alloc(1));
+    EMG_HIP(c, s64.alloc(2 * std::max<int64_t>(nnz, 1)));
+    EMG_HIP(c, s32.alloc(2 * std::max<int64_t>(nnz, 1)));
+    EMG_HIP(c, hipMemsetAsync(err.p, 0, sizeof(int32_t), s));
+    k_tmap<<<grid_for(n, 256), 256, 0, s>>>(n, ptr, idx, tmap.p, err.p);
+    k_symmetrize<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, tmap.p, reinterpret_cast<const double2*>(v),
+                                                    reinterpret_cast<double2*>(s64.p),
+                                                    reinterpret_cast<float2*>(s32.p));
+    EMG_HIP(c, hipGetLastError());
+    int32_t herr = 0;
+    EMG_HIP(c, hipMemcpyAsync(&herr, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipStreamSynchronize(s));
+    if (herr) return fail(c, EMG_ERR_ARG, "COCG needs a structurally symmetric pattern");
+    return EMG_OK;
+}
+
+// Fixed-order sum of the nblk x nv partials (layout [block][v], nv a power
+// of two <= 64) by one workgroup; the sum of v is left in red[v].
+__device__ void cg_sum_partials(const double2* part, int nblk, int nv, double2* red) {
+    const int tid = threadIdx.x;
+    const int parts = kCgThreads / nv;
+    const int v = tid % nv, pi = tid / nv;
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = pi; b < nblk; b += parts) {
+        const double2 t = part[int64_t(b) * nv + v];
+        s.x += t.x;
+        s.y += t.y;
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int h = parts / 2; h >= 1; h >>= 1) {
+        if (pi < h) {
+            red[tid].x += red[tid + h * nv].x;
+            red[tid].y += red[tid + h * nv].y;
+        }
+        __syncthreads();
+    }
+}
+
+// w = A_s z; then per row q = w + beta q and p = z + beta p (so that
+// q = A_s p without gathering p), and per-workgroup partial sums of p^T q.
+// Entries are visited four at a time so that their gathers overlap.
+template <typename R, int F>
+__global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
+                                                        const cplx<R>* __restrict__ gam,
+                                                        const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p,
+                                                        cplx<R>* __restrict__ q, CgState st) {
+    constexpr int G = 64 / F;
+    constexpr int W = kCgThreads / 64;
+    __shared__ double2 wsum[W][F];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int f = lane % F, g = lane / F;
+    const R kk = op.B ? k2[f] : R(0);
+    const cplx<R> bf = from_d<R>(st.beta[f]);
+    double2 dot = make_double2(0.0, 0.0);
+    const int64_t nw = int64_t(gridDim.x) * W;
+    auto entry = [&](int j) {
+        const int64_t c = op.idx[j];
+        cplx<R> a = op.E[j];
+        if (op.B) {
+            const cplx<R> b = op.B[j];
+            a.x -= kk * b.x;
+            a.y -= kk * b.y;
+        }
+        return cmul(a, z[c * F + f]);
+    };
+    for (int64_t row = int64_t(blockIdx.x) * W + wv; row < op.n; row += nw) {
+        cplx<R> acc = cmk<R>(R(0), R(0));
+        const int j1 = op.ptr[row + 1];
+        int j = op.ptr[row] + g;
+        for (; j + 3 * G < j1; j += 4 * G) {
+            const cplx<R> t0 = entry(j), t1 = entry(j + G), t2 = entry(j + 2 * G), t3 = entry(j + 3 * G);
+            acc = cadd(acc, cadd(cadd(t0, t1), cadd(t2, t3)));
+        }
+        for (; j < j1; j += G) acc = cadd(acc, entry(j));
+#pragma unroll
+        for (int o = 32; o >= F; o >>= 1) {
+            acc.x += __shfl_xor(acc.x, o);
+            acc.y += __shfl_xor(acc.y, o);
+        }
+        if (g == 0) {
+            if (op.sptr) {
+                for (int k = op.sptr[row]; k < op.sptr[row + 1]; ++k) {
+                    const int64_t c = op.scol[k];
+                    const cplx<R> gv = gam[op.sbc[k] * F + f];
+                    const cplx<R> coef = cmk<R>(gv.x * op.sval[k], gv.y * op.sval[k]);
+                    acc = cfma(coef, z[c * F + f], acc);
+                }
+            }
+            const int64_t o = row * F + f;
+            const cplx<R> qn = cfma(bf, q[o], acc);
+            const cplx<R> pn = cfma(bf, p[o], z[o]);
+            q[o] = qn;
+            p[o] = pn;
+            const double2 t = to_d(cmul(pn, qn));
+            dot.x += t.x;
+            dot.y += t.y;
+        }
+    }
+    if (g == 0) wsum[wv][f] = dot;
+    __syncthreads();
+    if (threadIdx.x < F) {
+        double2 sum = make_double2(0.0, 0.0);
+        for (int w = 0; w < W; ++w) {
+            sum.x += wsum[w][threadIdx.x].x;
+            sum.y += wsum[w][threadIdx.x].y;
+        }
+        st.part[int64_t(blockIdx.x) * F + threadIdx.x] = sum;
+    }
+}
+
+// alpha = rho / p^T q (one workgroup; partials of k_cg_spmv)
+template <int F>
+__global__ __launch_bounds__(kCgThreads) void k_cg_alpha(int nblk, CgState st) {
+    __shared__ double2 red[kCgThreads];
+    cg_sum_partials(st.part, nblk, F, red);
+    if (threadIdx.x < F) {
+        const int ff = threadIdx.x;
+        const double2 pq = red[ff];
+        double2 al = make_double2(0.0, 0.0);
+        if (!st.done[ff]) {
+            const double m2 = pq.x * pq.x + pq.y * pq.y;
+            if (m2 > 0 && isfinite(m2)) al = cdivd(st.rho[ff], pq);
+            else st.done[ff] = 1;  // breakdown: the refinement restarts this system
+        }
+        st.alpha[ff] = al;
+    }
+}
+
+// dx += alpha p; r -= alpha q; z = dinv o r; per-workgroup partial sums of
+// r^T z and ||r||^2.  init: only z and the sums of the starting residual
+// (dx, p and q zeroed).
+template <typename R, int F>
+__global__ __launch_bounds__(kCgThreads) void k_cg_update(int64_t nq, cplx<R>* __restrict__ p, cplx<R>* __restrict__ q,
+                                                          const cplx<R>* __restrict__ dinv, cplx<R>* __restrict__ r,
+                                                          cplx<R>* __restrict__ z, cplx<R>* __restrict__ dx, CgState st,
+                                                          int init) {
+    __shared__ double2 red[kCgThreads];
+    const int tid = threadIdx.x;
+    const int f = tid % F;  // blockDim and the grid stride are multiples of F
+    const cplx<R> al = init ? cmk<R>(R(0), R(0)) : from_d<R>(st.alpha[f]);
+    double2 rz = make_double2(0.0, 0.0);
+    double rr = 0.0;
+    for (int64_t k = int64_t(blockIdx.x) * kCgThreads + tid; k < nq; k += int64_t(gridDim.x) * kCgThreads) {
+        cplx<R> rv = r[k];
+        if (init) {
+            dx[k] = cmk<R>(R(0), R(0));
+            p[k] = cmk<R>(R(0), R(0));
+            q[k] = cmk<R>(R(0), R(0));
+        } else {
+            dx[k] = cfma(al, p[k], dx[k]);
+            const cplx<R> aq = cmul(al, q[k]);
+            rv.x -= aq.x;
+            rv.y -= aq.y;
+            r[k] = rv;
+        }
+        const cplx<R> zv = cmul(dinv[k], rv);
+        z[k] = zv;
+        const double2 t = to_d(cmul(rv, zv));
+        rz.x += t.x;
+        rz.y += t.y;
+        rr += cabs2(rv);
+    }
+    // fixed-order workgroup sums per f: threads f, f+F, f+2F, ...
+    const int parts = kCgThreads / F, pi = tid / F;
+    red[tid] = rz;
+    __syncthreads();
+    for (int h = parts / 2; h >= 1; h >>= 1) {
+        if (pi < h) {
+            red[tid].x += red[tid + h * F].x;
+            red[tid].y += red[tid + h * F].y;
+        }
+        __syncthreads();
+    }
+    if (tid < F) st.part[int64_t(blockIdx.x) * 2 * F + tid] = red[tid];
+    __syncthreads();
+    red[tid] = make_double2(rr, 0.0);
+    __syncthreads();
+    for (int h = parts / 2; h >= 1; h >>= 1) {
+        if (pi < h) red[tid].x += red[tid + h * F].x;
+        __syncthreads();
+    }
+    if (tid < F) st.part[int64_t(blockIdx.x) * 2 * F + F + tid] = red[tid];
+}
+
+// rho' and ||r||^2 from the partials of k_cg_update; beta = rho'/rho and
+// the stop flags (one workgroup)
+template <int F>
+__global__ __launch_bounds__(kCgThreads) void k_cg_beta(int nblk, CgState st, int init) {
+    __shared__ double2 red[kCgThreads];
+    const int tid = threadIdx.x;
+    cg_sum_partials(st.part, nblk, 2 * F, red);
+    if (tid < F) {
+        const double2 rho1 = red[tid];
+        const double rrv = red[F + tid].x;
+        st.rr[tid] = rrv;
+        int dn = init ? 0 : st.done[tid];
+        double2 be = make_double2(0.0, 0.0);
+        if (!(rrv <= st.tgt2[tid])) {
+            const double2 rho0 = st.rho[tid];
+            const double m2 = rho0.x * rho0.x + rho0.y * rho0.y;
+            if (!init && !dn) {
+                if (m2 > 0 && isfinite(m2)) be = cdivd(rho1, rho0);
+                else dn = 1;
+            }
+        } else {
+            dn = 1;
+        }
+        if (!isfinite(rrv)) dn = 1;
+        st.done[tid] = dn;
+        st.rho[tid] = rho1;
+        st.beta[tid] = be;
+    }
+}
+
+// r = cast(r64 / ||r64||) per system
+template <typename R, int F>
+__global__ void k_cast_scaled(int64_t nq, const double2* __restrict__ src, const double2* __restrict__ nrm,
+                              cplx<R>* __restrict__ dst) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const double n2 = nrm[k % F].x;
+    const double sc = n2 > 0 ? 1.0 / sqrt(n2) : 0.0;
+    dst[k] = from_d<R>(make_double2(src[k].x * sc, src[k].y * sc));
+}
+
+// x64 += ||r64|| dx per system
+template <typename R, int F>
+__global__ void k_cg_accum(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm,
+                           double2* __restrict__ x) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const double sc = sqrt(fmax(nrm[k % F].x, 0.0));
+    const double2 d = to_d(dx[k]);
+    x[k].x += sc * d.x;
+    x[k].y += sc * d.y;
+}
+
+template <typename R, int F>
+int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k2_h, const double2* gam_h, int nbc,
+             const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
+    hipStream_t s = c->stream;
+    const int64_t n = op.n, nq = n * F;
+    DBuf<cplx<R>> pv, q, r, z, dx, dinv;
+    DBuf<double2> r64, t64, nrm, bn, partial, rho, beta, alpha;
+    DBuf<double> rr, tgt2, k264;
+    DBuf<int> done;
+    DBuf<R> k2d;
+    DBuf<cplx<R>> gam;
+    DBuf<double2> gam64;
+    for (DBuf<cplx<R>>* b : {&pv, &q, &r, &z, &dx, &dinv}) EMG_HIP(c, b->alloc(nq));
+    EMG_HIP(c, r64.alloc(nq));
+    EMG_HIP(c, t64.alloc(nq));
+    for (DBuf<double2>* b : {&nrm, &bn, &rho, &beta, &alpha}) EMG_HIP(c, b->alloc(F));
+    EMG_HIP(c, rr.alloc(F));
+    EMG_HIP(c, tgt2.alloc(F));
+    EMG_HIP(c, done.alloc(F));
+    const int64_t nchunk = (nq + 511) / 512;
+    EMG_HIP(c, partial.alloc(std::max<int64_t>(nchunk * F, int64_t(kCgMaxBlocks) * 2 * F)));
+    EMG_HIP(c, k2d.alloc(F));
+    EMG_HIP(c, k264.alloc(F));
+    EMG_HIP(c, gam.alloc(std::max(1, nbc) * F));
+    EMG_HIP(c, gam64.alloc(std::max(1, nbc) * F));
+    {
+        std::vector<R> k2r(F);
+        std::vector<cplx<R>> gr(std::max(1, nbc) * F, cmk<R>(R(0), R(0)));
+        std::vector<double2> g64(std::max(1, nbc) * F, make_double2(0, 0));
+        for (int f = 0; f < F; ++f) k2r[f] = R(k2_h[f]);
+        for (int b = 0; b < nbc; ++b)
+            for (int f = 0; f < F; ++f) {
+                gr[b * F + f] = from_d<R>(gam_h[b * F + f]);
+                g64[b * F + f] = gam_h[b * F + f];
+            }
+        EMG_HIP(c, hipMemcpyAsync(k2d.p, k2r.data(), sizeof(R) * F, hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipMemcpyAsync(k264.p, k2_h, sizeof(double) * F, hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipMemcpyAsync(gam.p, gr.data(), sizeof(cplx<R>) * gr.size(), hipMemcpyHostToDevice, s));
+        EMG_HIP(c, hipMemcpyAsync(gam64.p, g64.data(), sizeof(double2) * g64.size(), hipMemcpyHostToDevice, s));
+    }
+    CgState cs;
+    cs.rho = rho.p;
+    cs.beta = beta.p;
+    cs.alpha = alpha.p;
+    cs.rr = rr.p;
+    cs.tgt2 = tgt2.p;
+    cs.done = done.p;
+    cs.part = partial.p;
+    const unsigned gq = grid_for(nq, 256);
+    const unsigned grow = grid_for(n, 4);
+    const unsigned gdots = grid_for(nchunk * 64, 256);
+    const unsigned ga = unsigned(std::min<int64_t>(kCgMaxBlocks, std::max<int64_t>(1, (n + 15) / 16)));
+    const unsigned gb = unsigned(std::min<int64_t>(kCgMaxBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
+    k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
+    EMG_HIP(c, hipGetLastError());
+    auto dots64 = [&](const double2* a, double2* out) {
+        k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, a, partial.p);
+        k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
+    };
+    std::vector<double2> hb(F), hr(F);
+    dots64(b64, bn.p);
+    EMG_HIP(c, hipMemcpyAsync(hb.data(), bn.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipStreamSynchronize(s));
+    std::vector<double> bnorm(F);
+    for (int f = 0; f < F; ++f) bnorm[f] = std::sqrt(std::max(hb[f].x, 0.0));
+    st.relres.assign(F, 1.0);
+    st.iters.assign(F, 0);
+    // stop flags land in pinned host memory; the graph of `blk` iterations
+    // is built on first use and freed on every exit path
+    struct Res {
+        int* h = nullptr;
+        double* hr = nullptr;
+        hipGraphExec_t g = nullptr;
+        ~Res() {
+            if (g) (void)hipGraphExecDestroy(g);
+            if (h) (void)hipHostFree(h);
+            if (hr) (void)hipHostFree(hr);
+        }
+    } res;
+    EMG_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&res.h), sizeof(int) * F, hipHostMallocDefault));
+    EMG_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&res.hr), sizeof(double) * F, hipHostMallocDefault));
+    int* hdone = res.h;
+    double* hrr = res.hr;
+    // an inner solve whose residual has not halved for this many iterations
+    // has stagnated (FP32 loss of bi-orthogonality): it is stopped and the
+    // refinement restarts it from the FP64 residual
+    const int stall = 2000;
+    hipGraphExec_t& graph = res.g;
+    const int blk = std::max(2, (P.check_every + 1) & ~1);
+    std::vector<double> ht(F);
+    for (int cyc = 0; cyc < P.max_cycles; ++cyc) {
+        k_spmv<double, F><<<grow, 256, 0, s>>>(op64, k264.p, gam64.p, x64, nullptr, t64.p);
+        k_sub<<<gq, 256, 0, s>>>(nq, b64, t64.p, r64.p);
+        dots64(r64.p, nrm.p);
+        EMG_HIP(c, hipMemcpyAsync(hr.data(), nrm.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+        EMG_HIP(c, hipStreamSynchronize(s));
+        EMG_HIP(c, hipGetLastError());
+        bool all = true;
+        for (int f = 0; f < F; ++f) {
+            const double rn = std::sqrt(std::max(hr[f].x, 0.0));
+            st.relres[f] = bnorm[f] > 0 ? rn / bnorm[f] : 0.0;
+            const bool ok = st.relres[f] <= P.tol;
+            if (!ok) all = false;
+            // inner target relative to the scaled residual (norm 1); a
+            // finished system gets a target it already meets
+            const double t = ok || rn == 0 ? 2.0 : std::max(P.inner_rel, 0.5 * P.tol * bnorm[f] / rn);
+            ht[f] = t * t;
+        }
+        st.cycles = cyc;
+        if (all) break;
+        EMG_HIP(c, hipMemcpyAsync(tgt2.p, ht.data(), sizeof(double) * F, hipMemcpyHostToDevice, s));
+        k_cast_scaled<R, F><<<gq, 256, 0, s>>>(nq, r64.p, nrm.p, r.p);
+        k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 1);
+        k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 1);
+        EMG_HIP(c, hipGetLastError());
+        if (!graph) {
+            // one graph = `blk` iterations
+            EMG_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < blk; ++i) {
+                k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, z.p, pv.p, q.p, cs);
+                k_cg_alpha<F><<<1, kCgThreads, 0, s>>>(int(ga), cs);
+                k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0);
+                k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 0);
+            }
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &g);
+            if (ec != hipSuccess) return fail(c, EMG_ERR_HIP, "COCG graph capture failed: %s", hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return fail(c, EMG_ERR_HIP, "COCG graph instantiate failed: %s", hipGetErrorString(ei));
+        }
+        int it = 0;
+        std::vector<int> active(F), it_best(F, 0);
+        std::vector<double> best(F, 1.0);
+        for (int f = 0; f < F; ++f) active[f] = !(st.relres[f] <= P.tol);
+        while (it < P.restart) {
+            EMG_HIP(c, hipGraphLaunch(graph, s));
+            it += blk;
+            for (int f = 0; f < F; ++f) st.iters[f] += active[f] ? blk : 0;
+            EMG_HIP(c, hipMemcpyAsync(hdone, done.p, sizeof(int) * F, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipMemcpyAsync(hrr, rr.p, sizeof(double) * F, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipStreamSynchronize(s));
+            bool fin = true, stop = false;
+            for (int f = 0; f < F; ++f) {
+                if (!hdone[f] && active[f]) {
+                    if (hrr[f] < 0.25 * best[f]) {
+                        best[f] = hrr[f];
+                        it_best[f] = it;
+                    } else if (it - it_best[f] > stall) {
+                        hdone[f] = 1;
+                        stop = true;
+                    }
+                }
+                if (hdone[f]) active[f] = 0;
+                else fin = false;
+            }
+            if (fin) break;
+            if (stop) EMG_HIP(c, hipMemcpyAsync(done.p, hdone, sizeof(int) * F, hipMemcpyHostToDevice, s));
+        }
+        if (getenv("EMG_COCG_TRACE")) {
+            std::fprintf(stderr, "[cocg] F=%d cycle %d: %d inner iterations, relres", F, cyc, it);
+            for (int f = 0; f < F; ++f) std::fprintf(stderr, " %.1e", st.relres[f]);
+            std::fprintf(stderr, "\n");
+        }
+        k_cg_accum<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm.p, x64);
+        EMG_HIP(c, hipGetLastError());
+        st.cycles = cyc + 1;
+    }
+    return EMG_OK;
+}
+
 template <typename R, int F>
 int run_method(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k2, const double2* gam, int nbc,
                const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
@@ -872,6 +1352,7 @@ int run_method(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double
         if (sizeof(R) != sizeof(double)) return fail(c, EMG_ERR_ARG, "IDR(s) requires FP64 (precision 0)");
         return run_idr<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
     }
+    if (P.method == 2) return run_cocg<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
     return run_batch<R, F>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
 }
 
@@ -943,11 +1424,23 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         op32.sbc = op64.sbc;
         op32.sval = c->s_val32.p;
     }
+    if (c->krylov_method == 2) {
+        // COCG iterates on the symmetric part: E - k2 (B + B^T)/2 + surface
+        // (E and the surface terms are symmetric already)
+        if (!c->have_bsym) {
+            const int rc = build_symmetric(c, n, c->nnz_red, c->red_ptr.p, c->red_idx.p, c->B64.p, c->Bs64, c->Bs32);
+            if (rc != EMG_OK) return rc;
+            c->have_bsym = true;
+        }
+        op32.B = reinterpret_cast<const float2*>(c->Bs32.p);
+    }
+    Op<double> op64s = op64;
+    if (c->krylov_method == 2) op64s.B = reinterpret_cast<const double2*>(c->Bs64.p);
     SolveParams P;
     P.tol = tol;
     P.restart = restart;
     P.max_cycles = max_cycles;
-    P.inner_rel = precision == 1 ? c->inner_rel32 : 1e-13;
+    P.inner_rel = (precision == 1 || c->krylov_method == 2) ? c->inner_rel32 : 1e-13;
     P.method = c->krylov_method;
     P.idr_s = c->idr_s;
     DBuf<double2> b64, x64, xt, coef;
@@ -969,7 +1462,7 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         BatchStats st;
         const int rc = precision == 1
                            ? dispatch_batch<float>(c, F, op32, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st)
-                           : dispatch_batch<double>(c, F, op64, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st);
+                           : dispatch_batch<double>(c, F, op64s, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st);
         if (rc != EMG_OK) return rc;
         if (overlap_out) {
             std::vector<double2> ov;
@@ -1026,6 +1519,7 @@ extern "C" int emg_csr_set(emg_ctx* c, int64_t n, const int32_t* indptr, const i
     EMG_HIP(c, hipGetLastError());
     EMG_HIP(c, hipStreamSynchronize(s));
     c->g_n = n;
+    c->g_have_sym = false;
     c->g_nnz = nnz;
     return EMG_OK;
 }
@@ -1052,11 +1546,21 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
     op32.idx = op64.idx;
     op32.E = reinterpret_cast<const float2*>(c->g_v32.p);
     op32.diag = op64.diag;
+    Op<double> op64s = op64;
+    if (c->krylov_method == 2) {
+        if (!c->g_have_sym) {
+            const int rc = build_symmetric(c, n, c->g_nnz, c->g_ptr.p, c->g_idx.p, c->g_v64.p, c->g_s64, c->g_s32);
+            if (rc != EMG_OK) return rc;
+            c->g_have_sym = true;
+        }
+        op64s.E = reinterpret_cast<const double2*>(c->g_s64.p);
+        op32.E = reinterpret_cast<const float2*>(c->g_s32.p);
+    }
     SolveParams P;
     P.tol = tol;
     P.restart = restart;
     P.max_cycles = max_cycles;
-    P.inner_rel = precision == 1 ? c->inner_rel32 : 1e-13;
+    P.inner_rel = (precision == 1 || c->krylov_method == 2) ? c->inner_rel32 : 1e-13;
     P.method = c->krylov_method;
     P.idr_s = c->idr_s;
     DBuf<double2> bin, b64, x64, xt;
@@ -1076,7 +1580,7 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
         double2 dummy = make_double2(0, 0);
         BatchStats st;
         rc = precision == 1 ? dispatch_batch<float>(c, F, op32, op64, k2.data(), &dummy, 0, b64.p, x64.p, P, st)
-                            : dispatch_batch<double>(c, F, op64, op64, k2.data(), &dummy, 0, b64.p, x64.p, P, st);
+                            : dispatch_batch<double>(c, F, op64s, op64, k2.data(), &dummy, 0, b64.p, x64.p, P, st);
         if (rc != EMG_OK) return rc;
         k_transpose<<<grid_for(n * F, 256), 256, 0, s>>>(n, F, x64.p, xt.p, 1);
         EMG_HIP(c, hipMemcpyAsync(x + 2 * done * n, xt.p, sizeof(double2) * n * F, hipMemcpyDeviceToHost, s));
@@ -1094,7 +1598,8 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
 
 extern "C" int emg_solver_config(emg_ctx* c, int method, int idr_s, double inner_rel32) {
     EMG_CHECK_CTX(c);
-    if (method != 0 && method != 1) return fail(c, EMG_ERR_ARG, "emg_solver_config: method must be 0 (GMRES) or 1 (IDR)");
+    if (method < 0 || method > 2)
+        return fail(c, EMG_ERR_ARG, "emg_solver_config: method must be 0 (GMRES), 1 (IDR) or 2 (COCG)");
     if (idr_s < 1 || idr_s > 16) return fail(c, EMG_ERR_ARG, "emg_solver_config: idr_s must be in [1, 16]");
     if (!(inner_rel32 > 0 && inner_rel32 < 1)) return fail(c, EMG_ERR_ARG, "emg_solver_config: inner_rel32 in (0,1)");
     c->krylov_method = method;

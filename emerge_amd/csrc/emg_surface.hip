@@ -302,6 +302,7 @@ extern "C" int emg_system_reduce(emg_ctx* c, int64_t n_keep, const int32_t* keep
     c->n_red = n_keep;
     c->nnz_red = tot;
     c->have_system = true;
+    c->have_bsym = false;
     c->have_surface = false;
     return EMG_OK;
 }

@@ -52,16 +52,24 @@ class SolveReport:
 
 
 class SolverHIP:
-    """GMRES on the GPU (FP64, or complex64 inside FP64 iterative refinement)."""
+    """Krylov solve on the GPU: GMRES(restart), IDR(s) or COCG on the
+    symmetric part inside FP64 refinement (method 'gmres' / 'idr' / 'cocg');
+    FP64, or complex64 inner iterations inside FP64 iterative refinement."""
 
     real_only: bool = False
     req_sorter: bool = False
     released_gil: bool = True
 
+    _METHODS = {"gmres": 0, "idr": 1, "cocg": 2}
+
     def __init__(self, pre: str = "", precision: str = "fp64", tol: float = 1e-10, restart: int = 150,
-                 max_cycles: int = 400, device: int = 0):
+                 max_cycles: int = 400, device: int = 0, method: str = "gmres", inner_rel: float | None = None):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if method not in self._METHODS:
+            raise ValueError(f"method must be one of {sorted(self._METHODS)}")
+        self.method = method
+        self.inner_rel = inner_rel if inner_rel is not None else (1e-2 if method == "cocg" else 1e-5)
         self.pre = pre
         self.own_preconditioner = True
         self.initialized = False
@@ -83,7 +91,8 @@ class SolverHIP:
         self.initialized = True
 
     def duplicate(self) -> "SolverHIP":
-        return self.__class__(self.pre, self.precision, self.tol, self.restart, self.max_cycles, self.device)
+        return self.__class__(self.pre, self.precision, self.tol, self.restart, self.max_cycles, self.device,
+                              self.method, self.inner_rel)
 
     def set_options(self, pivoting_threshold: float | None = None) -> None:
         """Pivoting thresholds do not apply to a Krylov solver."""
@@ -109,10 +118,11 @@ class SolverHIP:
         it = np.empty(1, dtype=np.int64)
         prec = 1 if self.precision == "mixed" else 0
         try:
+            self._ctx.call("emg_solver_config", self._METHODS[self.method], 4, float(self.inner_rel))
             self._ctx.call("emg_csr_solve", 1, ptr(bb), prec, float(self.tol), int(self.restart),
                            int(self.max_cycles), ptr(x), ptr(rel), ptr(it))
         except EmgError as e:
             raise SimulationError(str(e)) from e
-        aux = {"precision": self.precision, "iterations": str(int(it[0])), "relres": f"{rel[0]:.3e}",
+        aux = {"precision": self.precision, "method": self.method, "iterations": str(int(it[0])), "relres": f"{rel[0]:.3e}",
                "tolerance": str(self.tol)}
         return x, SolveReport(solver=str(self), exit_code=0, aux=aux, simtime=time.time() - t0)

@@ -281,9 +281,12 @@ class FrequencySweep:
               inner_rel32: float = 1e-5) -> SweepResult:
         """Solve every (frequency, active port) system.  method 'gmres' =
         GMRES(restart); 'idr' = IDR(idr_s) with a budget of `restart`
-        matvecs per FP64 refinement cycle."""
+        matvecs per FP64 refinement cycle; 'cocg' = COCG on the symmetric
+        part inside FP64 refinement, at most `restart` iterations per cycle,
+        each cycle reducing the residual by `inner_rel32`."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
-        self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1}[method], int(idr_s), float(inner_rel32))
+        self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s),
+                      float(inner_rel32))
         k2, gam, coef = self.batch_coefficients(freqs)
         nbatch = k2.shape[0]
         x = np.empty((nbatch, self.n_red), dtype=np.complex128) if keep_fields else None

@@ -152,3 +152,31 @@ def test_sweep_idr_matches_superlu(sweep, precision, s):
         o = wg_case.oracle_solution(float(f))
         assert np.abs(res.S[i] - o["S"]).max() <= 1e-6, (f, np.abs(res.S[i] - o["S"]).max())
     print(f"idr iters={res.iters.tolist()} relres={res.relres.max():.2e}")
+
+
+@pytest.mark.parametrize("precision,fblock", [("fp64", 1), ("fp64", 4), ("mixed", 4), ("mixed", 16)])
+def test_sweep_cocg_matches_superlu(sweep, precision, fblock):
+    print(f"cocg precision={precision} fblock={fblock}", flush=True)
+    freqs = np.array([8e9, 10e9, 12e9]) if fblock > 1 else np.array([10e9])
+    res = sweep.solve(freqs, precision=precision, tol=1e-8, fblock=fblock, restart=20000, max_cycles=30,
+                      method="cocg", inner_rel32=1e-2)
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = wg_case.oracle_solution(float(f))
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6, (f, np.abs(res.S[i] - o["S"]).max())
+    print(f"cocg iters={res.iters.tolist()} relres={res.relres.max():.2e}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solver_plugin_cocg(precision):
+    """SolverHIP (generic CSR) with COCG on the symmetric part of A."""
+    o = wg_case.oracle_solution(F0)
+    ids = o["ids"]
+    A = o["K"].tocsc()[np.ix_(ids, ids)]
+    b = (o["b"] + o["pv"][1])[ids]
+    s = SolverHIP("", precision=precision, tol=1e-10, method="cocg", restart=20000, max_cycles=40)
+    x, rep = s.solve(A, b, None, False, 0)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-10
+    ref = o["fields"][1][ids]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-6
+    assert rep.aux["method"] == "cocg"
