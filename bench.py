@@ -57,13 +57,14 @@ def cpu_baseline(n_tets_sample: int = 100_000) -> dict:
                       f"scipy COO->CSR, {dt:.1f} s"}
 
 
-def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method: str = "gmres",
-                idr_s: int = 4, restart: int = 100) -> dict:
+def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method: str = "cocg",
+                idr_s: int = 4, restart: int = 20000) -> dict:
     """WR-90 waveguide (Kuhn 11x5x18, 5,940 tets, 41,118 DOFs), TE10 ports,
-    frequency points from a 201-point 8-12 GHz grid, both ports excited per
-    point.  Plain FP64 GPU path: FP64 GMRES one (frequency, port) system at a
-    time.  Mixed path: complex64 GMRES in FP64 refinement, `fblock` systems per
-    batch.  A point counts once every port reached ||b-Ax||/||b|| <= 1e-8."""
+    frequency points from the 201-point 8-12 GHz grid, both ports excited
+    per point.  Plain FP64 GPU path: the FP64 solver one (frequency, port)
+    system at a time.  Mixed path: complex64 inner iterations in FP64
+    refinement, `fblock` systems per batch.  A point counts once every port
+    reached ||b-Ax||/||b|| <= 1e-8 (FP64 residual on the device)."""
     from emerge_amd.sweep import FrequencySweep
     from emerge_amd.workloads import waveguide, waveguide_bcs
     prob = waveguide()
@@ -135,17 +136,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=60, help="cube cells per side")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sweep-plain", type=int, default=3, help="FP64 plain-path frequency points")
-    ap.add_argument("--sweep-mixed", type=int, default=32, help="mixed batched frequency points")
-    ap.add_argument("--fblock", type=int, default=16)
+    ap.add_argument("--sweep-plain", type=int, default=4, help="FP64 plain-path frequency points")
+    ap.add_argument("--sweep-mixed", type=int, default=201, help="mixed batched frequency points")
+    ap.add_argument("--fblock", type=int, default=32, help="systems per mixed batch")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-only", action="store_true", help="profile helper: run only the solve sweep")
-    ap.add_argument("--method", default="gmres", choices=["gmres", "idr", "cocg"])
+    ap.add_argument("--method", default="cocg", choices=["gmres", "idr", "cocg"])
     ap.add_argument("--idr-s", type=int, default=4)
-    ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--restart", type=int, default=None,
+                    help="GMRES dimension / IDR matvecs per cycle / COCG iterations per cycle")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="counter calibration: only run 3 device-to-device copies of this many MiB")
     args = ap.parse_args()
+    if args.restart is None:
+        args.restart = {"gmres": 100, "idr": 30000, "cocg": 20000}[args.method]
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU sharding is "
               "documented, not launched", file=sys.stderr)
@@ -204,6 +208,13 @@ def main() -> int:
 
     T, V, nnz = m.n_tets, m.n_nodes, dm.nnz
     algo = assembly_algorithmic_bytes(T, V, nnz)
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    # this workload (copy-calibrated); only for the 60^3 cube it was taken on
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "assembly_cube60_hbm_traffic.json")
+    if args.n == 60 and os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = int(json.load(fh)["traffic_bytes"])
     achieved = algo / (ms * 1e-3) / 1e9
     value = T / (ms * 1e-3) / 1e6
 
@@ -243,7 +254,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
         "sweep": sweep,

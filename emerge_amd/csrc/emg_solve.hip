@@ -882,7 +882,8 @@ int run_idr(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k
 // per-system stop flags between blocks.
 
 constexpr int kCgThreads = 1024;    // threads per workgroup of the COCG kernels
-constexpr int kCgMaxBlocks = 2048;  // grid cap (partial sums per kernel)
+constexpr int kCgMaxBlocks = 1024;  // grid cap of the SpMV (partial sums per kernel)
+constexpr int kCgUpdBlocks = 256;   // grid cap of the update pass
 
 struct CgState {
     double2* rho;    // [F] r^T z
@@ -957,6 +958,7 @@ __device__ void cg_sum_partials(const double2* part, int nblk, int nv, double2* 
     const int parts = kCgThreads / nv;
     const int v = tid % nv, pi = tid / nv;
     double2 s = make_double2(0.0, 0.0);
+#pragma unroll 8
     for (int b = pi; b < nblk; b += parts) {
         const double2 t = part[int64_t(b) * nv + v];
         s.x += t.x;
@@ -975,65 +977,84 @@ __device__ void cg_sum_partials(const double2* part, int nblk, int nv, double2* 
 
 // w = A_s z; then per row q = w + beta q and p = z + beta p (so that
 // q = A_s p without gathering p), and per-workgroup partial sums of p^T q.
-// Entries are visited four at a time so that their gathers overlap.
+// One wavefront per row; a lane covers V consecutive systems of the batch
+// (one V-wide gather per entry) and every G-th entry of the row.  The p^T q
+// sums of a wavefront accumulate in LDS (few live registers: more resident
+// wavefronts hide the gather latency).
 template <typename R, int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
                                                         const cplx<R>* __restrict__ gam,
                                                         const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p,
                                                         cplx<R>* __restrict__ q, CgState st) {
-    constexpr int G = 64 / F;
+    constexpr int V = F >= 4 ? 4 : F;  // systems per lane
+    constexpr int FG = F / V;          // lanes per entry
+    constexpr int G = 64 / FG;         // entries in flight per wavefront
     constexpr int W = kCgThreads / 64;
     __shared__ double2 wsum[W][F];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int f = lane % F, g = lane / F;
-    const R kk = op.B ? k2[f] : R(0);
-    const cplx<R> bf = from_d<R>(st.beta[f]);
-    double2 dot = make_double2(0.0, 0.0);
-    const int64_t nw = int64_t(gridDim.x) * W;
-    auto entry = [&](int j) {
-        const int64_t c = op.idx[j];
-        cplx<R> a = op.E[j];
-        if (op.B) {
-            const cplx<R> b = op.B[j];
-            a.x -= kk * b.x;
-            a.y -= kk * b.y;
-        }
-        return cmul(a, z[c * F + f]);
-    };
-    for (int64_t row = int64_t(blockIdx.x) * W + wv; row < op.n; row += nw) {
-        cplx<R> acc = cmk<R>(R(0), R(0));
-        const int j1 = op.ptr[row + 1];
-        int j = op.ptr[row] + g;
-        for (; j + 3 * G < j1; j += 4 * G) {
-            const cplx<R> t0 = entry(j), t1 = entry(j + G), t2 = entry(j + 2 * G), t3 = entry(j + 3 * G);
-            acc = cadd(acc, cadd(cadd(t0, t1), cadd(t2, t3)));
-        }
-        for (; j < j1; j += G) acc = cadd(acc, entry(j));
+    const int fg = lane % FG, g = lane / FG, f0 = fg * V;
+    if (g == 0) {
 #pragma unroll
-        for (int o = 32; o >= F; o >>= 1) {
-            acc.x += __shfl_xor(acc.x, o);
-            acc.y += __shfl_xor(acc.y, o);
+        for (int v = 0; v < V; ++v) wsum[wv][f0 + v] = make_double2(0.0, 0.0);
+    }
+    // XCD-aware rows: workgroups are dealt round-robin over the 8 XCDs;
+    // XCD x walks rows [x n/8, (x+1) n/8) so that its L2 holds the part of
+    // z those rows gather (the grid is a multiple of 8)
+    const int64_t per = (op.n + 7) / 8;
+    const int64_t xcd = blockIdx.x % 8;
+    const int64_t r0 = xcd * per, r1 = min(op.n, r0 + per);
+    const int64_t nw = int64_t(gridDim.x / 8) * W;
+    for (int64_t row = r0 + int64_t(blockIdx.x / 8) * W + wv; row < r1; row += nw) {
+        cplx<R> acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = cmk<R>(R(0), R(0));
+        const int j1 = op.ptr[row + 1];
+        for (int j = op.ptr[row] + g; j < j1; j += G) {
+            const int64_t c = op.idx[j];
+            const cplx<R> e = op.E[j];
+            const cplx<R> b = op.B ? op.B[j] : cmk<R>(R(0), R(0));
+            const cplx<R>* zc = z + c * F + f0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const R kk = op.B ? k2[f0 + v] : R(0);
+                acc[v] = cfma(cmk<R>(e.x - kk * b.x, e.y - kk * b.y), zc[v], acc[v]);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+#pragma unroll
+            for (int o = 32; o >= FG; o >>= 1) {
+                acc[v].x += __shfl_xor(acc[v].x, o);
+                acc[v].y += __shfl_xor(acc[v].y, o);
+            }
         }
         if (g == 0) {
             if (op.sptr) {
                 for (int k = op.sptr[row]; k < op.sptr[row + 1]; ++k) {
                     const int64_t c = op.scol[k];
-                    const cplx<R> gv = gam[op.sbc[k] * F + f];
-                    const cplx<R> coef = cmk<R>(gv.x * op.sval[k], gv.y * op.sval[k]);
-                    acc = cfma(coef, z[c * F + f], acc);
+                    const R sv = op.sval[k];
+                    const int bc = op.sbc[k];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const cplx<R> gv = gam[bc * F + f0 + v];
+                        acc[v] = cfma(cmk<R>(gv.x * sv, gv.y * sv), z[c * F + f0 + v], acc[v]);
+                    }
                 }
             }
-            const int64_t o = row * F + f;
-            const cplx<R> qn = cfma(bf, q[o], acc);
-            const cplx<R> pn = cfma(bf, p[o], z[o]);
-            q[o] = qn;
-            p[o] = pn;
-            const double2 t = to_d(cmul(pn, qn));
-            dot.x += t.x;
-            dot.y += t.y;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int64_t o = row * F + f0 + v;
+                const cplx<R> bf = from_d<R>(st.beta[f0 + v]);
+                const cplx<R> qn = cfma(bf, q[o], acc[v]);
+                const cplx<R> pn = cfma(bf, p[o], z[o]);
+                q[o] = qn;
+                p[o] = pn;
+                const double2 t = to_d(cmul(pn, qn));
+                wsum[wv][f0 + v].x += t.x;
+                wsum[wv][f0 + v].y += t.y;
+            }
         }
     }
-    if (g == 0) wsum[wv][f] = dot;
     __syncthreads();
     if (threadIdx.x < F) {
         double2 sum = make_double2(0.0, 0.0);
@@ -1223,8 +1244,8 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     const unsigned gq = grid_for(nq, 256);
     const unsigned grow = grid_for(n, 4);
     const unsigned gdots = grid_for(nchunk * 64, 256);
-    const unsigned ga = unsigned(std::min<int64_t>(kCgMaxBlocks, std::max<int64_t>(1, (n + 15) / 16)));
-    const unsigned gb = unsigned(std::min<int64_t>(kCgMaxBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
+    const unsigned ga = unsigned(8 * std::min<int64_t>(kCgMaxBlocks / 8, std::max<int64_t>(1, (n + 127) / 128)));
+    const unsigned gb = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
     k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
     EMG_HIP(c, hipGetLastError());
     auto dots64 = [&](const double2* a, double2* out) {
@@ -1259,6 +1280,10 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     // has stagnated (FP32 loss of bi-orthogonality): it is stopped and the
     // refinement restarts it from the FP64 residual
     const int stall = 2000;
+    // EMG_COCG_GRAPH=0: plain launches instead of graph replay (same kernels;
+    // for tools that do not follow graph launches)
+    const char* ge = getenv("EMG_COCG_GRAPH");
+    const bool use_graph = !(ge && ge[0] == '0');
     hipGraphExec_t& graph = res.g;
     const int blk = std::max(2, (P.check_every + 1) & ~1);
     std::vector<double> ht(F);
@@ -1287,15 +1312,18 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 1);
         k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 1);
         EMG_HIP(c, hipGetLastError());
-        if (!graph) {
-            // one graph = `blk` iterations
-            EMG_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        auto iterations = [&]() {
             for (int i = 0; i < blk; ++i) {
                 k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, z.p, pv.p, q.p, cs);
                 k_cg_alpha<F><<<1, kCgThreads, 0, s>>>(int(ga), cs);
                 k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0);
                 k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 0);
             }
+        };
+        if (!graph && use_graph) {
+            // one graph = `blk` iterations
+            EMG_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            iterations();
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(s, &g);
             if (ec != hipSuccess) return fail(c, EMG_ERR_HIP, "COCG graph capture failed: %s", hipGetErrorString(ec));
@@ -1308,7 +1336,9 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         std::vector<double> best(F, 1.0);
         for (int f = 0; f < F; ++f) active[f] = !(st.relres[f] <= P.tol);
         while (it < P.restart) {
-            EMG_HIP(c, hipGraphLaunch(graph, s));
+            if (use_graph) EMG_HIP(c, hipGraphLaunch(graph, s));
+            else iterations();
+            EMG_HIP(c, hipGetLastError());
             it += blk;
             for (int f = 0; f < F; ++f) st.iters[f] += active[f] ? blk : 0;
             EMG_HIP(c, hipMemcpyAsync(hdone, done.p, sizeof(int) * F, hipMemcpyDeviceToHost, s));

@@ -224,16 +224,31 @@ __global__ void k_red_fill(int64_t N, const int32_t* __restrict__ ptr, const int
     if (r >= N) return;
     const int32_t rr = keep[r];
     if (rr < 0) return;
-    int o = rptr[rr];
-    rdiag[rr] = -1;
+    const int o0 = rptr[rr];
+    int o = o0;
     for (int j = ptr[r]; j < ptr[r + 1]; ++j) {
         const int32_t cc = keep[idx[j]];
         if (cc < 0) continue;
         ridx[o] = cc;
         rmap[o] = j;
-        if (cc == rr) rdiag[rr] = o;
         ++o;
     }
+    // the kept ids may come in any order: sort the row by reduced column
+    // (insertion sort; rows hold a few dozen entries)
+    for (int a = o0 + 1; a < o; ++a) {
+        const int32_t cc = ridx[a], jj = rmap[a];
+        int z = a - 1;
+        while (z >= o0 && ridx[z] > cc) {
+            ridx[z + 1] = ridx[z];
+            rmap[z + 1] = rmap[z];
+            --z;
+        }
+        ridx[z + 1] = cc;
+        rmap[z + 1] = jj;
+    }
+    rdiag[rr] = -1;
+    for (int a = o0; a < o; ++a)
+        if (ridx[a] == rr) rdiag[rr] = a;
 }
 
 __global__ void k_gather_vals(int64_t n, const int32_t* __restrict__ map, const double2* __restrict__ src,
@@ -257,8 +272,8 @@ extern "C" int emg_system_reduce(emg_ctx* c, int64_t n_keep, const int32_t* keep
     std::vector<int32_t> keep(c->n_field, -1);
     for (int64_t k = 0; k < n_keep; ++k) {
         const int32_t r = keep_ids[k];
-        if (r < 0 || r >= c->n_field || keep[r] >= 0 || (k > 0 && keep_ids[k - 1] >= r))
-            return fail(c, EMG_ERR_ARG, "emg_system_reduce: keep ids must be strictly increasing in [0, N)");
+        if (r < 0 || r >= c->n_field || keep[r] >= 0)
+            return fail(c, EMG_ERR_ARG, "emg_system_reduce: keep ids must be distinct and in [0, N)");
         keep[r] = int32_t(k);
     }
     hipStream_t st = c->stream;

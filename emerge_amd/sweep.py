@@ -77,7 +77,10 @@ class FrequencySweep:
     """Device-resident E, B, surface terms and forcing of one mesh."""
 
     def __init__(self, mesh, basis, eps_r: np.ndarray, inv_mu_r: np.ndarray, boundary_conditions: list,
-                 device: int = 0, ctx: Context | None = None):
+                 device: int = 0, ctx: Context | None = None, dof_order: str = "spatial"):
+        if dof_order not in ("spatial", "natural"):
+            raise ValueError("dof_order must be 'spatial' or 'natural'")
+        self.dof_order = dof_order
         self.mesh = mesh
         self.basis = basis
         self.ctx = ctx if ctx is not None else Context(device)
@@ -101,12 +104,32 @@ class FrequencySweep:
             ids.append(f.tri_to_field[:, t].reshape(-1))
         return np.unique(np.concatenate(ids))
 
+    def spatial_order(self, ids: np.ndarray) -> np.ndarray:
+        """`ids` reordered along a Morton curve of the DOF positions (edge
+        midpoints, face centroids; both DOFs of an entity stay adjacent), so
+        that consecutive reduced rows -- and each XCD's contiguous share of
+        them -- couple to nearby columns."""
+        m, f = self.mesh, self.basis
+        ec = 0.5 * (m.nodes[:, m.edges[0]] + m.nodes[:, m.edges[1]])
+        fc = (m.nodes[:, m.tris[0]] + m.nodes[:, m.tris[1]] + m.nodes[:, m.tris[2]]) / 3.0
+        pos = np.concatenate([ec, fc, ec, fc], axis=1)[:, ids]
+        lo = pos.min(axis=1, keepdims=True)
+        h = max(float(np.median(m.edge_lengths)), 1e-300)
+        q = np.clip(np.floor((pos - lo) / h), 0, 1023).astype(np.int64)
+        key = np.zeros(ids.shape[0], dtype=np.int64)
+        for b in range(10):
+            for d in range(3):
+                key |= ((q[d] >> b) & 1) << (3 * b + d)
+        return ids[np.lexsort((ids, key))]
+
     def _setup(self) -> None:
         f, m = self.basis, self.mesh
         N = f.n_field
         keep_mask = np.ones(N, dtype=bool)
         keep_mask[self.pec_dofs()] = False
         self.solve_ids = np.nonzero(keep_mask)[0]
+        if self.dof_order == "spatial":
+            self.solve_ids = self.spatial_order(self.solve_ids)
         red = np.full(N, -1, dtype=np.int64)
         red[self.solve_ids] = np.arange(self.solve_ids.shape[0])
         self.red = red

@@ -99,11 +99,13 @@ int emg_values_get(emg_ctx* ctx, double* E_vals, double* B_vals);
 
 /* ------------------------------------------------------- reduced system */
 
-/* Keep only the DOFs in keep_ids (strictly increasing): the PEC elimination
- * A[ix_(ids, ids)] of SolveRoutine.solve (E/solver.py:1266-1269) with
- * solve_ids from ASM/assembler.py:274-307, 406-407.  Builds the reduced CSR
- * pattern on the device and gathers the assembled E and B values into it
- * (FP64 and FP32 copies). */
+/* Keep only the DOFs in keep_ids (distinct; reduced index k = keep_ids[k],
+ * so the order of keep_ids is the order of the reduced system): the PEC
+ * elimination A[ix_(ids, ids)] of SolveRoutine.solve (E/solver.py:1266-1269)
+ * with solve_ids from ASM/assembler.py:274-307, 406-407, optionally
+ * permuted.  Builds the reduced CSR pattern (columns sorted per row) on the
+ * device and gathers the assembled E and B values into it (FP64 and FP32
+ * copies). */
 int emg_system_reduce(emg_ctx* ctx, int64_t n_keep, const int32_t* keep_ids);
 /* Copy the reduced pattern (indptr n_keep + 1, indices nnz); either may be
  * NULL.  nnz receives the reduced nonzero count. */
