@@ -975,28 +975,71 @@ __device__ void cg_sum_partials(const double2* part, int nblk, int nv, double2* 
     }
 }
 
+// Two consecutive complex values as one 16-byte load.
+__device__ __forceinline__ void load_pair(const float2* p, float2& a, float2& b) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    a = make_float2(v.x, v.y);
+    b = make_float2(v.z, v.w);
+}
+__device__ __forceinline__ void load_pair(const double2* p, double2& a, double2& b) {
+    a = p[0];
+    b = p[1];
+}
+
+// (ar + i ai)[v] += (E[j] - k2[v] B[j]) z[idx[j]][f0 + v] for the V systems
+// of a lane (real and imaginary parts in separate arrays: they stay in
+// registers for complex128 too)
+template <typename R, int F, int V>
+__device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restrict__ z, const R (&kk)[V], int f0,
+                                         int j, R (&ar)[V], R (&ai)[V]) {
+    const int64_t c = op.idx[j];
+    const cplx<R> e = op.E[j];
+    const cplx<R> b = op.B ? op.B[j] : cmk<R>(R(0), R(0));
+    const cplx<R>* zc = z + c * F + f0;
+    cplx<R> zv[V];
+    if constexpr (V % 2 == 0) {
+#pragma unroll
+        for (int v = 0; v < V; v += 2) load_pair(zc + v, zv[v], zv[v + 1]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) zv[v] = zc[v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const R mr = e.x - kk[v] * b.x, mi = e.y - kk[v] * b.y;
+        ar[v] += mr * zv[v].x - mi * zv[v].y;
+        ai[v] += mr * zv[v].y + mi * zv[v].x;
+    }
+}
+
 // w = A_s z; then per row q = w + beta q and p = z + beta p (so that
 // q = A_s p without gathering p), and per-workgroup partial sums of p^T q.
 // One wavefront per row; a lane covers V consecutive systems of the batch
-// (one V-wide gather per entry) and every G-th entry of the row.  The p^T q
-// sums of a wavefront accumulate in LDS (few live registers: more resident
-// wavefronts hide the gather latency).
+// (V/2 16-byte gathers per matrix entry) and every G-th entry of the row,
+// two entries per step so that their gathers overlap.  The butterfly
+// reduction leaves each row sum in all G lanes of its systems, so the row
+// end (surface terms, q/p update, p^T q) is spread one system per lane, and
+// those lanes load their q, p, z before the gathers.
 template <typename R, int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
                                                         const cplx<R>* __restrict__ gam,
                                                         const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p,
                                                         cplx<R>* __restrict__ q, CgState st) {
-    constexpr int V = F >= 4 ? 4 : F;  // systems per lane
-    constexpr int FG = F / V;          // lanes per entry
-    constexpr int G = 64 / FG;         // entries in flight per wavefront
+    constexpr int VM = sizeof(R) == 4 ? 8 : 4;  // 8 complex64 or 4 complex128: 64 bytes per lane
+    constexpr int V = F >= VM ? VM : F;         // systems per lane
+    constexpr int FG = F / V;                   // lanes per entry
+    constexpr int G = 64 / FG;                  // entries in flight per wavefront (G >= V)
     constexpr int W = kCgThreads / 64;
     __shared__ double2 wsum[W][F];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int fg = lane % FG, g = lane / FG, f0 = fg * V;
-    if (g == 0) {
+    const bool own = g < V;                     // row end of system f0 + g
+    const int fo = f0 + (own ? g : 0);
+    R kk[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) wsum[wv][f0 + v] = make_double2(0.0, 0.0);
-    }
+    for (int v = 0; v < V; ++v) kk[v] = op.B ? k2[f0 + v] : R(0);
+    const cplx<R> bf = from_d<R>(st.beta[fo]);
+    double2 dot = make_double2(0.0, 0.0);
     // XCD-aware rows: workgroups are dealt round-robin over the 8 XCDs;
     // XCD x walks rows [x n/8, (x+1) n/8) so that its L2 holds the part of
     // z those rows gather (the grid is a multiple of 8)
@@ -1005,56 +1048,60 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     const int64_t r0 = xcd * per, r1 = min(op.n, r0 + per);
     const int64_t nw = int64_t(gridDim.x / 8) * W;
     for (int64_t row = r0 + int64_t(blockIdx.x / 8) * W + wv; row < r1; row += nw) {
-        cplx<R> acc[V];
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = cmk<R>(R(0), R(0));
-        const int j1 = op.ptr[row + 1];
-        for (int j = op.ptr[row] + g; j < j1; j += G) {
-            const int64_t c = op.idx[j];
-            const cplx<R> e = op.E[j];
-            const cplx<R> b = op.B ? op.B[j] : cmk<R>(R(0), R(0));
-            const cplx<R>* zc = z + c * F + f0;
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                const R kk = op.B ? k2[f0 + v] : R(0);
-                acc[v] = cfma(cmk<R>(e.x - kk * b.x, e.y - kk * b.y), zc[v], acc[v]);
-            }
+        const int64_t o = row * F + fo;
+        R qx = R(0), qy = R(0), px = R(0), py = R(0), zx = R(0), zy = R(0);
+        if (own) {
+            const cplx<R> qv = q[o], pv = p[o], zv = z[o];
+            qx = qv.x;
+            qy = qv.y;
+            px = pv.x;
+            py = pv.y;
+            zx = zv.x;
+            zy = zv.y;
         }
+        R ar[V], ai[V], br[V], bi[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) ar[v] = ai[v] = br[v] = bi[v] = R(0);
+        const int j1 = op.ptr[row + 1];
+        int j = op.ptr[row] + g;
+        for (; j + G < j1; j += 2 * G) {
+            cg_entry<R, F, V>(op, z, kk, f0, j, ar, ai);
+            cg_entry<R, F, V>(op, z, kk, f0, j + G, br, bi);
+        }
+        if (j < j1) cg_entry<R, F, V>(op, z, kk, f0, j, ar, ai);
+        // the pick is done on scalars: a select of vector-type structs is
+        // lowered to an indexed stack load for complex128
+        R ax = R(0), ay = R(0);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
+            R tx = ar[v] + br[v], ty = ai[v] + bi[v];
 #pragma unroll
-            for (int o = 32; o >= FG; o >>= 1) {
-                acc[v].x += __shfl_xor(acc[v].x, o);
-                acc[v].y += __shfl_xor(acc[v].y, o);
+            for (int off = 32; off >= FG; off >>= 1) {
+                tx += __shfl_xor(tx, off);
+                ty += __shfl_xor(ty, off);
             }
+            ax = (v == g) ? tx : ax;
+            ay = (v == g) ? ty : ay;
         }
-        if (g == 0) {
+        cplx<R> a = cmk<R>(ax, ay);
+        if (own) {
             if (op.sptr) {
                 for (int k = op.sptr[row]; k < op.sptr[row + 1]; ++k) {
-                    const int64_t c = op.scol[k];
+                    const cplx<R> gv = gam[op.sbc[k] * F + fo];
                     const R sv = op.sval[k];
-                    const int bc = op.sbc[k];
-#pragma unroll
-                    for (int v = 0; v < V; ++v) {
-                        const cplx<R> gv = gam[bc * F + f0 + v];
-                        acc[v] = cfma(cmk<R>(gv.x * sv, gv.y * sv), z[c * F + f0 + v], acc[v]);
-                    }
+                    a = cfma(cmk<R>(gv.x * sv, gv.y * sv), z[int64_t(op.scol[k]) * F + fo], a);
                 }
             }
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                const int64_t o = row * F + f0 + v;
-                const cplx<R> bf = from_d<R>(st.beta[f0 + v]);
-                const cplx<R> qn = cfma(bf, q[o], acc[v]);
-                const cplx<R> pn = cfma(bf, p[o], z[o]);
-                q[o] = qn;
-                p[o] = pn;
-                const double2 t = to_d(cmul(pn, qn));
-                wsum[wv][f0 + v].x += t.x;
-                wsum[wv][f0 + v].y += t.y;
-            }
+            const cplx<R> qn = cfma(bf, cmk<R>(qx, qy), a);
+            const cplx<R> pn = cfma(bf, cmk<R>(px, py), cmk<R>(zx, zy));
+            q[o] = qn;
+            p[o] = pn;
+            const double2 t = to_d(cmul(pn, qn));
+            dot.x += t.x;
+            dot.y += t.y;
         }
     }
+    if (own) wsum[wv][fo] = dot;
     __syncthreads();
     if (threadIdx.x < F) {
         double2 sum = make_double2(0.0, 0.0);
