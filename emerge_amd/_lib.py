@@ -50,6 +50,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_port_probe_set": (_c_int, [_p, ctypes.c_int32, _p, _p, _p, _p, _p]),
     "emg_port_norms_get": (_c_int, [_p, _p]),
     "emg_solver_config": (_c_int, [_p, _c_int, _c_int, ctypes.c_double]),
+    "emg_port_points_set": (_c_int, [_p, ctypes.c_int32, _p, _p, _p, _p, _p]),
     "emg_csr_set": (_c_int, [_p, _i64, _p, _p, _p]),
     "emg_csr_solve": (_c_int, [_p, ctypes.c_int32, _p, _c_int, ctypes.c_double, _c_int, _c_int, _p, _p, _p]),
 }

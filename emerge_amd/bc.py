@@ -160,7 +160,10 @@ class RectangularWaveguide(PortBC):
 
 
 class LumpedPort(PortBC):
-    """Uniform-field sheet port of impedance Z0_port (MW/microwave_bc.py:1018-1163)."""
+    """Uniform-field sheet port of impedance Z0_port (MW/microwave_bc.py:1018-1163);
+    its S-parameters come from the voltage along lines over the port height."""
+
+    v_integration = True
 
     def __init__(self, tri_ids, port_number: int, width: float, height: float, direction,
                  power: float = 1.0, Z0_port: float = 50.0):

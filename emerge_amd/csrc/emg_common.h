@@ -118,15 +118,15 @@ struct emg_ctx {
     int32_t n_forcing = 0;
     emg::DBuf<double> forcing;    // 2 x n_forcing x n_red
 
-    // port probes for S-parameter overlaps (mode-overlap ports)
+    // port probes for S-parameter overlaps (mode-overlap and voltage-line ports)
     int32_t n_ports = 0;
-    int64_t n_probe_tris = 0;
-    std::vector<int64_t> port_tri_start;  // n_ports + 1
-    emg::DBuf<int32_t> probe_tet;         // owning tet per probe triangle
-    emg::DBuf<double> probe_pts;          // [n][6][3]
-    emg::DBuf<double2> probe_shape;       // [n][6][3]
-    emg::DBuf<double> probe_wa;           // [n][6] quadrature weight x area
-    std::vector<double> port_norm;        // sum w a |shape|^2 per port
+    int64_t n_probe_pts = 0;
+    std::vector<int64_t> port_pt_start;   // n_ports + 1
+    emg::DBuf<int32_t> probe_tet;         // owning tet per probe point
+    emg::DBuf<double> probe_pts;          // [n][3]
+    emg::DBuf<double2> probe_shape;       // [n][3] complex vector c
+    emg::DBuf<double> probe_wa;           // [n] weight
+    std::vector<double> port_norm;        // sum w |c|^2 per port
 
     // Krylov method of the solves (emg_solver_config)
     int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part

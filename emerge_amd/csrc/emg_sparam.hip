@@ -1,13 +1,19 @@
-// Port mode-overlap integrals for S-parameter extraction.
+// Port overlap integrals for S-parameter extraction.
 //
-// For a mode-overlap port the reference integrates, over the port triangles
-// with the 6-point Dunavant rule, (E - Q e_mode) . conj(e_mode) / (2 Z_mode)
-// (sparam_field_power / sparam_mode_power, MW/sparam.py:71-137, through
-// surface_integral, E/mth/integrals.py:60-92), with E interpolated from the
-// solution in the tet that owns each point (ned2_tet_interp,
-// E/elements/ned2_interp.py:141-301).  Because e_mode = q(k0) * shape(x), the
-// device only needs O = sum w A (E . conj(shape)) per (system, port) and the
-// k0-independent N = sum w A |shape|^2; the host combines the scalars.
+// A probe is a list of points per port, each with the tet that owns it, a
+// complex vector c and a weight w; the device returns, per solved system and
+// port, O = sum_i w_i (E(p_i) . conj(c_i)) with E interpolated from the
+// solution in the owner tet (ned2_tet_interp, E/elements/ned2_interp.py:
+// 141-301), and the k0-independent N = sum_i w_i |c_i|^2.
+//   * Mode-overlap ports (sparam_field_power / sparam_mode_power,
+//     MW/sparam.py:71-137, through surface_integral, E/mth/integrals.py:
+//     60-92): the 6 Dunavant points of every port triangle, c = the mode
+//     shape (e_mode = q(k0) shape), w = weight x area.
+//   * Voltage-line ports (MW/microwave_3d.py:1271-1292 with
+//     Line.line_integral, E/coord.py:149-153): the points of each line,
+//     c = the line step vector, w = the composite-rule weight / number of
+//     lines, so O is the mean line voltage.
+// The host combines the scalars.
 #include <algorithm>
 
 #include "emg_common.h"
@@ -17,16 +23,16 @@ namespace {
 
 using namespace emg;
 
-// partial[k][f] = sum_q wa[k][q] * (E_f(p_kq) . conj(shape_kq)) for probe
-// triangle k, E_f from the reduced solution x[row][f] (eliminated DOFs = 0).
-__global__ void k_port_overlap(int64_t ntri, int F, const int32_t* __restrict__ owner,
+// partial[k][f] = w[k] * (E_f(p_k) . conj(c_k)) for probe point k, E_f from
+// the reduced solution x[row][f] (eliminated DOFs = 0).
+__global__ void k_port_overlap(int64_t npts, int F, const int32_t* __restrict__ owner,
                                const double* __restrict__ pts, const double2* __restrict__ shape,
                                const double* __restrict__ wa, const double* __restrict__ xyz, int64_t V,
                                const int32_t* __restrict__ tets, int64_t T, const int32_t* __restrict__ t2f,
                                const int32_t* __restrict__ keep, const double2* __restrict__ x,
                                double2* __restrict__ partial) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= ntri) return;
+    if (k >= npts) return;
     const int t = owner[k];
     int w[4];
     double xv[4], yv[4], zv[4];
@@ -72,8 +78,8 @@ __global__ void k_port_overlap(int64_t ntri, int F, const int32_t* __restrict__ 
             const int r = keep[t2f[i * T + t]];
             dof[i] = r >= 0 ? x[int64_t(r) * F + f] : make_double2(0.0, 0.0);
         }
-        for (int q = 0; q < 6; ++q) {
-            const double px = pts[(k * 6 + q) * 3 + 0], py = pts[(k * 6 + q) * 3 + 1], pz = pts[(k * 6 + q) * 3 + 2];
+        {
+            const double px = pts[k * 3 + 0], py = pts[k * 3 + 1], pz = pts[k * 3 + 2];
             double Fv[4];
             for (int j = 0; j < 4; ++j) Fv[j] = aa[j] + bb[j] * px + cc[j] * py + dd[j] * pz;
             double2 E[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
@@ -111,11 +117,11 @@ __global__ void k_port_overlap(int64_t ntri, int F, const int32_t* __restrict__ 
             }
             double2 dotv = make_double2(0.0, 0.0);
             for (int d = 0; d < 3; ++d) {
-                const double2 sh = shape[(k * 6 + q) * 3 + d];
-                dotv.x += E[d].x * sh.x + E[d].y * sh.y;  // E . conj(shape)
+                const double2 sh = shape[k * 3 + d];
+                dotv.x += E[d].x * sh.x + E[d].y * sh.y;  // E . conj(c)
                 dotv.y += E[d].y * sh.x - E[d].x * sh.y;
             }
-            const double wq = wa[k * 6 + q];
+            const double wq = wa[k];
             acc.x += wq * dotv.x;
             acc.y += wq * dotv.y;
         }
@@ -123,7 +129,7 @@ __global__ void k_port_overlap(int64_t ntri, int F, const int32_t* __restrict__ 
     }
 }
 
-// out[f][p] = sum over triangles k of port p (in order) of partial[k][f]
+// out[f][p] = sum over the points k of port p (in order) of partial[k][f]
 __global__ void k_port_sum(int n_ports, int F, const int64_t* __restrict__ start, const double2* __restrict__ partial,
                            double2* __restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,42 +143,38 @@ __global__ void k_port_sum(int n_ports, int F, const int64_t* __restrict__ start
     out[int64_t(f) * n_ports + p] = s;
 }
 
-// per-triangle k0-independent mode norm sum_q wa |shape|^2 (as .x)
-__global__ void k_port_norm_tri(int64_t ntri, const double2* __restrict__ shape, const double* __restrict__ wa,
-                                double2* __restrict__ part) {
+// per-point k0-independent norm w |c|^2 (as .x)
+__global__ void k_port_norm_pt(int64_t npts, const double2* __restrict__ shape, const double* __restrict__ wa,
+                               double2* __restrict__ part) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= ntri) return;
-    double tri = 0.0;
-    for (int q = 0; q < 6; ++q) {
-        double v = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const double2 z = shape[(k * 6 + q) * 3 + d];
-            v += z.x * z.x + z.y * z.y;
-        }
-        tri += wa[k * 6 + q] * v;
+    if (k >= npts) return;
+    double v = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double2 z = shape[k * 3 + d];
+        v += z.x * z.x + z.y * z.y;
     }
-    part[k] = make_double2(tri, 0.0);
+    part[k] = make_double2(wa[k] * v, 0.0);
 }
 
 }  // namespace
 
 namespace emg {
 
-// overlap[f][p] = sum over port p's triangles (in order) of partial[k][f]
+// overlap[f][p] = sum over port p's points (in order) of partial[k][f]
 int port_overlaps(emg_ctx* c, int F, const double2* x, std::vector<double2>& out) {
     out.assign(size_t(F) * c->n_ports, make_double2(0.0, 0.0));
     if (c->n_ports == 0) return EMG_OK;
     DBuf<double2> part;
-    EMG_HIP(c, part.alloc(c->n_probe_tris * F));
-    k_port_overlap<<<unsigned((c->n_probe_tris + 63) / 64), 64, 0, c->stream>>>(
-        c->n_probe_tris, F, c->probe_tet.p, c->probe_pts.p, c->probe_shape.p, c->probe_wa.p, c->xyz.p, c->n_nodes,
+    EMG_HIP(c, part.alloc(c->n_probe_pts * F));
+    k_port_overlap<<<unsigned((c->n_probe_pts + 63) / 64), 64, 0, c->stream>>>(
+        c->n_probe_pts, F, c->probe_tet.p, c->probe_pts.p, c->probe_shape.p, c->probe_wa.p, c->xyz.p, c->n_nodes,
         c->tets.p, c->n_tets, c->t2f.p, c->keep.p, x, part.p);
     EMG_HIP(c, hipGetLastError());
     DBuf<double2> dout;
     DBuf<int64_t> dstart;
     EMG_HIP(c, dout.alloc(size_t(F) * c->n_ports));
     EMG_HIP(c, dstart.alloc(c->n_ports + 1));
-    EMG_HIP(c, hipMemcpyAsync(dstart.p, c->port_tri_start.data(), sizeof(int64_t) * (c->n_ports + 1),
+    EMG_HIP(c, hipMemcpyAsync(dstart.p, c->port_pt_start.data(), sizeof(int64_t) * (c->n_ports + 1),
                               hipMemcpyHostToDevice, c->stream));
     k_port_sum<<<unsigned((c->n_ports * F + 63) / 64), 64, 0, c->stream>>>(c->n_ports, F, dstart.p, part.p, dout.p);
     EMG_HIP(c, hipGetLastError());
@@ -183,43 +185,43 @@ int port_overlaps(emg_ctx* c, int F, const double2* x, std::vector<double2>& out
 
 }  // namespace emg
 
-extern "C" int emg_port_probe_set(emg_ctx* c, int32_t n_ports, const int64_t* tri_counts, const int32_t* owner_tet,
-                                  const double* pts, const double* shape, const double* wa) {
+extern "C" int emg_port_points_set(emg_ctx* c, int32_t n_ports, const int64_t* point_counts,
+                                   const int32_t* owner_tet, const double* pts, const double* vec, const double* w) {
     EMG_CHECK_CTX(c);
-    if (c->n_tets == 0) return emg::fail(c, EMG_ERR_STATE, "emg_port_probe_set: no mesh");
-    if (n_ports < 1 || !tri_counts || !owner_tet || !pts || !shape || !wa)
-        return emg::fail(c, EMG_ERR_ARG, "emg_port_probe_set: bad arguments");
+    if (c->n_tets == 0) return emg::fail(c, EMG_ERR_STATE, "emg_port_points_set: no mesh");
+    if (n_ports < 1 || !point_counts || !owner_tet || !pts || !vec || !w)
+        return emg::fail(c, EMG_ERR_ARG, "emg_port_points_set: bad arguments");
     int64_t n = 0;
-    c->port_tri_start.assign(n_ports + 1, 0);
+    std::vector<int64_t> start(n_ports + 1, 0);
     for (int p = 0; p < n_ports; ++p) {
-        if (tri_counts[p] < 1) return emg::fail(c, EMG_ERR_ARG, "emg_port_probe_set: empty port");
-        n += tri_counts[p];
-        c->port_tri_start[p + 1] = n;
+        if (point_counts[p] < 1) return emg::fail(c, EMG_ERR_ARG, "emg_port_points_set: empty port");
+        n += point_counts[p];
+        start[p + 1] = n;
     }
     for (int64_t k = 0; k < n; ++k)
         if (owner_tet[k] < 0 || owner_tet[k] >= c->n_tets)
-            return emg::fail(c, EMG_ERR_ARG, "emg_port_probe_set: owner tet out of range");
+            return emg::fail(c, EMG_ERR_ARG, "emg_port_points_set: owner tet out of range");
     hipStream_t s = c->stream;
     EMG_HIP(c, c->probe_tet.alloc(n));
-    EMG_HIP(c, c->probe_pts.alloc(18 * n));
-    EMG_HIP(c, c->probe_shape.alloc(18 * n));
-    EMG_HIP(c, c->probe_wa.alloc(6 * n));
+    EMG_HIP(c, c->probe_pts.alloc(3 * n));
+    EMG_HIP(c, c->probe_shape.alloc(3 * n));
+    EMG_HIP(c, c->probe_wa.alloc(n));
     EMG_HIP(c, hipMemcpyAsync(c->probe_tet.p, owner_tet, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-    EMG_HIP(c, hipMemcpyAsync(c->probe_pts.p, pts, sizeof(double) * 18 * n, hipMemcpyHostToDevice, s));
-    EMG_HIP(c, hipMemcpyAsync(c->probe_shape.p, shape, sizeof(double2) * 18 * n, hipMemcpyHostToDevice, s));
-    EMG_HIP(c, hipMemcpyAsync(c->probe_wa.p, wa, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->probe_pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->probe_shape.p, vec, sizeof(double2) * 3 * n, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->probe_wa.p, w, sizeof(double) * n, hipMemcpyHostToDevice, s));
     EMG_HIP(c, hipStreamSynchronize(s));
     c->n_ports = n_ports;
-    c->n_probe_tris = n;
-    // k0-independent mode norms: per triangle on the device, then per port in order
+    c->n_probe_pts = n;
+    c->port_pt_start = start;
+    // k0-independent norms: per point on the device, then per port in order
     DBuf<double2> part, dout;
     DBuf<int64_t> dstart;
     EMG_HIP(c, part.alloc(n));
     EMG_HIP(c, dout.alloc(n_ports));
     EMG_HIP(c, dstart.alloc(n_ports + 1));
-    EMG_HIP(c, hipMemcpyAsync(dstart.p, c->port_tri_start.data(), sizeof(int64_t) * (n_ports + 1),
-                              hipMemcpyHostToDevice, s));
-    k_port_norm_tri<<<unsigned((n + 63) / 64), 64, 0, s>>>(n, c->probe_shape.p, c->probe_wa.p, part.p);
+    EMG_HIP(c, hipMemcpyAsync(dstart.p, start.data(), sizeof(int64_t) * (n_ports + 1), hipMemcpyHostToDevice, s));
+    k_port_norm_pt<<<unsigned((n + 63) / 64), 64, 0, s>>>(n, c->probe_shape.p, c->probe_wa.p, part.p);
     k_port_sum<<<unsigned((n_ports + 63) / 64), 64, 0, s>>>(n_ports, 1, dstart.p, part.p, dout.p);
     EMG_HIP(c, hipGetLastError());
     std::vector<double2> h(n_ports);
@@ -228,6 +230,25 @@ extern "C" int emg_port_probe_set(emg_ctx* c, int32_t n_ports, const int64_t* tr
     c->port_norm.assign(n_ports, 0.0);
     for (int p = 0; p < n_ports; ++p) c->port_norm[p] = h[p].x;
     return EMG_OK;
+}
+
+extern "C" int emg_port_probe_set(emg_ctx* c, int32_t n_ports, const int64_t* tri_counts, const int32_t* owner_tet,
+                                  const double* pts, const double* shape, const double* wa) {
+    EMG_CHECK_CTX(c);
+    if (n_ports < 1 || !tri_counts || !owner_tet || !pts || !shape || !wa)
+        return emg::fail(c, EMG_ERR_ARG, "emg_port_probe_set: bad arguments");
+    int64_t n = 0;
+    std::vector<int64_t> cnt(n_ports);
+    for (int p = 0; p < n_ports; ++p) {
+        if (tri_counts[p] < 1) return emg::fail(c, EMG_ERR_ARG, "emg_port_probe_set: empty port");
+        n += tri_counts[p];
+        cnt[p] = 6 * tri_counts[p];
+    }
+    // the 6 Dunavant points of a triangle share its owner tet
+    std::vector<int32_t> own(6 * n);
+    for (int64_t k = 0; k < n; ++k)
+        for (int q = 0; q < 6; ++q) own[6 * k + q] = owner_tet[k];
+    return emg_port_points_set(c, n_ports, cnt.data(), own.data(), pts, shape, wa);
 }
 
 extern "C" int emg_port_norms_get(emg_ctx* c, double* norms) {

@@ -216,33 +216,79 @@ class FrequencySweep:
             self.ctx.call("emg_forcing_set", len(self.ports), frow.shape[0], ptr(frow), ptr(fseg),
                           fcon_a.shape[0], ptr(fcon_a))
 
-        # S-parameter probes: mode-overlap ports (the lumped-port voltage line
-        # integral is not implemented on the device)
+        # S-parameter probes: Dunavant points of the port faces for mode-overlap
+        # ports, voltage-line points for lumped ports
         self.port_norms = None
-        if self.ports and all(isinstance(p, bcs.RectangularWaveguide) for p in self.ports):
-            counts, owners, P_, SH, WA = [], [], [], [], []
+        if self.ports:
+            counts, owners, P_, C_, W_ = [], [], [], [], []
             for p in self.ports:
-                t = p.tri_ids
-                counts.append(t.shape[0])
-                owners.append(m.tri_to_tet[0, t])
-                pts = dunavant_points(m.nodes, m.tris, t)
+                if getattr(p, "v_integration", False):
+                    own, pts, vec, w = self.voltage_line_probe(p)
+                else:
+                    t = p.tri_ids
+                    pts = dunavant_points(m.nodes, m.tris, t)  # (n, 6, 3)
+                    own = np.repeat(m.tri_to_tet[0, t], 6)
+                    shp = np.asarray(p.mode_shape(pts[..., 0].reshape(-1), pts[..., 1].reshape(-1),
+                                                  pts[..., 2].reshape(-1)), dtype=np.complex128)
+                    vec = shp.reshape(3, -1).T
+                    v = m.nodes[:, m.tris[:, t]]
+                    cr = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], axis=0)
+                    area = np.sqrt(cr[0] ** 2 + cr[1] ** 2 + cr[2] ** 2) / 2  # calc_area, E/mth/optimized.py:276-291
+                    w = (_DUN_W[None, :] * area[:, None]).reshape(-1)
+                    pts = pts.reshape(-1, 3)
+                counts.append(pts.shape[0])
+                owners.append(own)
                 P_.append(pts)
-                shp = np.asarray(p.mode_shape(pts[..., 0].reshape(-1), pts[..., 1].reshape(-1),
-                                              pts[..., 2].reshape(-1)), dtype=np.complex128)
-                SH.append(shp.reshape(3, -1).T.reshape(-1, 6, 3))
-                v = m.nodes[:, m.tris[:, t]]
-                cr = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], axis=0)
-                area = np.sqrt(cr[0] ** 2 + cr[1] ** 2 + cr[2] ** 2) / 2  # calc_area, E/mth/optimized.py:276-291
-                WA.append(_DUN_W[None, :] * area[:, None])
+                C_.append(vec)
+                W_.append(w)
             cnt = np.ascontiguousarray(counts, dtype=np.int64)
             own = np.ascontiguousarray(np.concatenate(owners), dtype=np.int32)
             pts = np.ascontiguousarray(np.concatenate(P_), dtype=np.float64)
-            shp = np.ascontiguousarray(np.concatenate(SH), dtype=np.complex128)
-            wa = np.ascontiguousarray(np.concatenate(WA), dtype=np.float64)
-            self.ctx.call("emg_port_probe_set", len(self.ports), ptr(cnt), ptr(own), ptr(pts), ptr(shp), ptr(wa))
+            vec = np.ascontiguousarray(np.concatenate(C_), dtype=np.complex128)
+            w = np.ascontiguousarray(np.concatenate(W_), dtype=np.float64)
+            self.ctx.call("emg_port_points_set", len(self.ports), ptr(cnt), ptr(own), ptr(pts), ptr(vec), ptr(w))
             norms = np.empty(len(self.ports))
             self.ctx.call("emg_port_norms_get", ptr(norms))
             self.port_norms = norms
+
+    def voltage_line_probe(self, port, npts: int = 21):
+        """Probe points of a voltage-line port.  Lines (MW/microwave_3d.py:
+        336-372): from every port-face node with the smallest coordinate along
+        the port direction, `npts` equally spaced points over the port height.
+        Line integral (E/coord.py:24-61, 149-153): the composite 3-point rule on
+        the samples of E . step is, per section (y0, y1, y2), the combination
+        (y0 + 4 y1 + y2) / 3; the port voltage is the mean over the lines.
+        Each point belongs to the last face-adjacent tet that contains it
+        (E/elements/ned2_interp.py:141-301 tolerances)."""
+        m = self.mesh
+        d = port.direction
+        ids = np.unique(m.tris[:, port.tri_ids])
+        P = m.nodes[:, ids]
+        dot = P[0] * d[0] + P[1] * d[1] + P[2] * d[2]
+        starts = P[:, dot == dot.min()]
+        nl = starts.shape[1]
+        lines = [np.stack([np.linspace(s0[k], s0[k] + d[k] * port.height, npts) for k in range(3)], axis=1)
+                 for s0 in starts.T]
+        pts = np.concatenate(lines)                                  # (nl*npts, 3)
+        coef = np.zeros(npts)
+        for i in range(0, npts - 2, 2):
+            coef[i:i + 3] += (1.0 / 3.0, 4.0 / 3.0, 1.0 / 3.0)
+        w = np.tile(coef, nl) / nl
+        step = np.concatenate([np.repeat((ln[1] - ln[0])[None, :], npts, axis=0) for ln in lines])
+        vec = step.astype(np.complex128)
+        tets = m.get_face_tets(port.tri_ids)
+        owner = np.full(pts.shape[0], -1, dtype=np.int64)
+        X = pts.T
+        for it in tets:
+            v = m.nodes[:, m.tets[:, it]]
+            Bl = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=1)
+            loc = np.linalg.solve(Bl, X - v[:, 0:1])
+            inside = ((loc[0] + loc[1] + loc[2]) <= 1.00000001) & (loc[0] >= -1e-6) & (loc[1] >= -1e-6) \
+                & (loc[2] >= -1e-6)
+            owner[inside] = it
+        if np.any(owner < 0):
+            raise ValueError(f"port {port.port_number}: voltage line leaves the face-adjacent tets")
+        return owner, pts, vec, w
 
     # -------------------------------------------------------------- accessors
     def reduced_pattern(self):
@@ -278,16 +324,24 @@ class FrequencySweep:
         return np.ascontiguousarray(k2), np.ascontiguousarray(gam[:, :nb] if nb else gam), np.ascontiguousarray(coef)
 
     def s_matrix(self, overlaps: np.ndarray, freqs: np.ndarray, caps: bool = True) -> np.ndarray:
-        """S from the device overlaps (mode-overlap ports): for receiving port
-        q and active port j, field_p = (conj(q_m) O - [q==j] |q_m|^2 N_q) / 2Z,
-        mode_p = |q_m|^2 N_q / 2Z, S_qj = field_p / mode_p / sqrt(P_j)
-        (MW/microwave_3d.py:1193-1211, 1294-1303; MW/sparam.py:71-137)."""
+        """S from the device overlaps, receiving port q, active port j
+        (MW/microwave_3d.py:1193-1211, 1250-1303; MW/sparam.py:71-137).
+        Mode-overlap port: field_p = (conj(q_m) O - [q==j] |q_m|^2 N_q) / 2Z,
+        mode_p = |q_m|^2 N_q / 2Z, S_qj = field_p / mode_p / sqrt(P_j).
+        Voltage-line port (O = mean line voltage V, V_q = sqrt(2 P_q Z0_q)):
+        S_qj = ((V - [q==j] V_q) / V_q) / sqrt(P_j)."""
         npt = len(self.ports)
         S = np.zeros((freqs.shape[0], npt, npt), dtype=np.complex128)
         for i, f in enumerate(freqs):
             k0 = 2 * np.pi * f / C0
             for j, act in enumerate(self.ports):
                 for q, prb in enumerate(self.ports):
+                    if getattr(prb, "v_integration", False):
+                        V = overlaps[i * npt + j, q]
+                        a = prb.voltage
+                        b = V - a if q == j else V
+                        S[i, q, j] = (b / a) / np.sqrt(act.power)
+                        continue
                     qm = prb.mode_factor(k0)
                     Zm = prb.Zmode(k0)
                     O = overlaps[i * npt + j, q]

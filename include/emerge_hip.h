@@ -172,7 +172,18 @@ int emg_sweep_solve(emg_ctx* ctx, int64_t nbatch, const double* k2, const double
 int emg_port_probe_set(emg_ctx* ctx, int32_t n_ports, const int64_t* tri_counts,
                        const int32_t* owner_tet, const double* pts, const double* shape,
                        const double* wa);
-/* k0-independent mode norms N_p = sum w A |shape|^2 (n_ports doubles). */
+/* General probes: per port point_counts[p] points (ports concatenated), each
+ * with its owner tet, coordinates ([n][3]), complex vector c ([n][3]) and
+ * weight w ([n]); emg_sweep_solve then writes O = sum w (E(p) . conj(c)) per
+ * system and port.  Voltage-line ports use the points of their integration
+ * lines with c = the line step and w = the composite-rule weight / number of
+ * lines, so O is the port voltage of Line.line_integral averaged over the
+ * lines (MW/microwave_3d.py:1271-1292, E/coord.py:24-61, 149-153).  Replaces
+ * any earlier probe set (emg_port_probe_set is its triangle form). */
+int emg_port_points_set(emg_ctx* ctx, int32_t n_ports, const int64_t* point_counts, const int32_t* owner_tet,
+                        const double* pts, const double* vec, const double* w);
+/* k0-independent norms N_p = sum w |c|^2 (n_ports doubles; for mode-overlap
+ * ports sum w A |shape|^2). */
 int emg_port_norms_get(emg_ctx* ctx, double* norms);
 
 /* Krylov method for emg_sweep_solve / emg_csr_solve: 0 = GMRES(restart)

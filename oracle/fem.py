@@ -185,10 +185,59 @@ def surface_integral(nodes, tri_vertices, function, constants=None, gq_order=4):
     return tot
 
 
+def gauss3_composite(x, y):
+    """E/coord.py:24-61: per section (y0, y1, y2) the quadratic through the
+    samples evaluated at the 3 Gauss nodes of [-1, 1] and weighted."""
+    x = np.asarray(x, dtype=np.complex128)
+    y = np.asarray(y, dtype=np.complex128)
+    if x.size % 2 == 0:
+        raise ValueError("Number of samples must be odd")
+    xi = np.sqrt(3 / 5)
+    nodes = np.array([-xi, 0.0, +xi])
+    weights = np.array([5 / 9, 8 / 9, 5 / 9])
+    total = 0.0
+    for i in range(0, x.size - 2, 2):
+        y0, y1, y2 = y[i:i + 3]
+        a = y0 * 0.5 - y1 + 0.5 * y2
+        b = -y0 * 0.5 + 0.5 * y2
+        c = y1
+        total += np.dot(weights, a * nodes ** 2 + b * nodes + c)
+    return total
+
+
+def voltage_lines(mesh, port, npts=21):
+    """MW/microwave_3d.py:336-372 (Line.from_points, E/coord.py:100-117): one
+    line per port-face node at the minimum coordinate along the port
+    direction, from that node over the port height."""
+    points = np.unique(mesh.tris[:, port.tri_ids].ravel())
+    xs, ys, zs = mesh.nodes[0, points], mesh.nodes[1, points], mesh.nodes[2, points]
+    ax = port.direction
+    dot = xs * ax[0] + ys * ax[1] + zs * ax[2]
+    start_id = np.argwhere(dot == np.min(dot)).flatten()
+    lines = []
+    for x, y, z in zip(xs[start_id], ys[start_id], zs[start_id]):
+        start = np.array([x, y, z])
+        end = start + ax * port.height
+        lines.append((np.linspace(start[0], end[0], npts), np.linspace(start[1], end[1], npts),
+                      np.linspace(start[2], end[2], npts)))
+    return lines
+
+
+def line_integral(line, fieldf):
+    """Line.line_integral (E/coord.py:63-84, 149-153)."""
+    xs, ys, zs = line
+    dxs, dys, dzs = xs[1:] - xs[:-1], ys[1:] - ys[:-1], zs[1:] - zs[:-1]
+    dl = np.sqrt(dxs ** 2 + dys ** 2 + dzs ** 2)
+    ll = np.concatenate((np.array([0.0]), np.cumsum(dl)))
+    Ex, Ey, Ez = fieldf(xs, ys, zs)
+    return gauss3_composite(ll, Ex * dxs[0] + Ey * dys[0] + Ez * dzs[0])
+
+
 def s_parameters(mesh, tet_to_field, fields: dict, ports: list, k0: float,
                  cap_single=True, cap_col=True, recip=False) -> np.ndarray:
-    """MW/microwave_3d.py:1167-1235 with _compute_s_data (:1250-1303) for
-    waveguide (mode-power) ports; mu_r = 1 on the port faces."""
+    """MW/microwave_3d.py:1167-1235 with _compute_s_data (:1250-1303):
+    waveguide (mode-power) ports and voltage-line (lumped) ports; mu_r = 1 on
+    the port faces."""
     numbers = [p.port_number for p in ports]
     Sp = np.zeros((len(ports), len(ports)), dtype=np.complex128)
     for ja, act in enumerate(ports):
@@ -201,6 +250,14 @@ def s_parameters(mesh, tet_to_field, fields: dict, ports: list, k0: float,
             def fieldf(x, y, z):
                 return np.array(ned2_tet_interp(np.array([x, y, z]), sol, mesh, tet_to_field, port_tets))
 
+            if getattr(bc, "v_integration", False):
+                V = sum(line_integral(ln, fieldf) for ln in voltage_lines(mesh, bc)) / len(voltage_lines(mesh, bc))
+                a = bc.voltage
+                b = V - bc.voltage if bc.active else V
+                a_sig = a * np.sqrt(1 / (2 * bc.Z0) + 0j)
+                b_sig = b * np.sqrt(1 / (2 * bc.Z0) + 0j)
+                Sp[ip, ja] = (b_sig / a_sig) / np.sqrt(act.power)
+                continue
             Q = 1 if bc.active else 0
             Zm = bc.Zmode(k0)
             const = np.ones(tv.shape[1])

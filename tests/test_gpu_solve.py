@@ -181,3 +181,24 @@ def test_solver_plugin_cocg(precision):
     ref = o["fields"][1][ids]
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-6
     assert rep.aux["method"] == "cocg"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cavity_lumped_port_matches_superlu(precision):
+    """Lumped port (voltage-line S11) in a lossy PEC cavity: residual and S11
+    against SuperLU with the oracle's S extraction."""
+    from tests import cavity_case
+    c = cavity_case.case()
+    sw = FrequencySweep(c["mesh"], c["basis"], c["eps"], c["inv_mu"], c["prod_bcs"])
+    freqs = np.array([6e9, 9e9])
+    res = sw.solve(freqs, precision=precision, tol=1e-8, fblock=2, restart=20000, max_cycles=40,
+                   method="cocg", inner_rel32=1e-2)
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = cavity_case.oracle_solution(float(f))
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6, (f, res.S[i], o["S"])
+        x = res.fields[(i, 1)]
+        ids = o["ids"]
+        ref = o["fields"][1][ids]
+        assert np.linalg.norm(x[ids] - ref) / np.linalg.norm(ref) <= 1e-6
+    print(f"cavity S11={res.S[:, 0, 0]} iters={res.iters.ravel().tolist()}")

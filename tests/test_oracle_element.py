@@ -130,3 +130,30 @@ def test_matinv_general_path():
     inv = cc.matinv(M)
     for t in range(4):
         np.testing.assert_allclose(inv[:, :, t] @ M[:, :, t], np.eye(3), atol=1e-12)
+
+
+def test_line_integral_closed_forms():
+    """gauss3_composite over equally spaced samples integrates cubics along a
+    line exactly: constant, linear and cubic E_z over the height."""
+    from oracle.fem import line_integral
+    h = 5e-3
+    line = (np.full(21, 1e-3), np.full(21, 2e-3), np.linspace(0.0, h, 21))
+    zero = lambda z: np.zeros_like(z)
+    assert abs(line_integral(line, lambda x, y, z: (zero(z), zero(z), np.ones_like(z))) - h) < 1e-18
+    assert abs(line_integral(line, lambda x, y, z: (zero(z), zero(z), z)) - h ** 2 / 2) < 1e-18
+    assert abs(line_integral(line, lambda x, y, z: (zero(z), zero(z), z ** 3)) - h ** 4 / 4) < 1e-18
+    # a field across the line contributes nothing
+    assert abs(line_integral(line, lambda x, y, z: (np.ones_like(z), np.ones_like(z), zero(z)))) == 0
+
+
+def test_voltage_lines_follow_the_port_direction():
+    """Lines start at the port-face nodes with the smallest coordinate along
+    the port direction and run over the port height (MW/microwave_3d.py:336-372)."""
+    from tests import cavity_case
+    from oracle.fem import voltage_lines
+    c = cavity_case.case()
+    lines = voltage_lines(c["mesh"], c["o_ports"][0])
+    assert len(lines) == 2                       # the two floor nodes of a one-cell-wide sheet
+    for xs, ys, zs in lines:
+        assert np.allclose(xs, cavity_case.PORT["x"]) and np.allclose(zs[[0, -1]], [0.0, 5e-3])
+        assert ys[0] in (5e-3, 7.5e-3)
