@@ -61,13 +61,19 @@ class RobinBC(BoundaryCondition):
 
 
 class AbsorbingBoundary(RobinBC):
+    """First- or second-order absorbing boundary (MW/microwave_bc.py:291-344;
+    order 2 with the coefficient sets A-E, default order 2 type B as the
+    reference).  Order 2 adds the surface curl/div correction with factor
+    1j c2 / k0 (ASM/assembler.py:342-348)."""
     o2coeffs = {"A": (1.0, -0.5), "B": (1.00023, -0.51555), "C": (1.03084, -0.73631),
                 "D": (1.06103, -0.84883), "E": (1.12500, -1.00000)}
 
-    def __init__(self, tri_ids, order: int = 1, abctype: str = "B"):
+    def __init__(self, tri_ids, order: int = 2, abctype: str = "B"):
         super().__init__(tri_ids)
-        if order != 1:
-            raise NotImplementedError("second-order absorbing boundaries are not implemented on the GPU path")
+        if order not in (1, 2):
+            raise ValueError("order must be 1 or 2")
+        if abctype not in self.o2coeffs:
+            raise ValueError(f"abctype must be one of {sorted(self.o2coeffs)}")
         self.order = order
         self.abctype = abctype
 
@@ -75,6 +81,10 @@ class AbsorbingBoundary(RobinBC):
         if self.order == 1:
             return 1j * k0
         return 1j * k0 * self.o2coeffs[self.abctype][0]
+
+    def abc2_coefficient(self, k0):
+        """Factor of the second-order correction term."""
+        return 1j * self.o2coeffs[self.abctype][1] / k0
 
 
 class PortBC(RobinBC):

@@ -181,6 +181,109 @@ __global__ void k_tri_terms(int64_t ntri, const int32_t* __restrict__ tv, const 
     }
 }
 
+// Second-order ABC correction per boundary triangle, without its
+// frequency factor cf = 1j c2 / k0 (ASM/robin_abc_order2.py:188-311,
+// ASM/assembler.py:342-348): L_i L_j |A| sum_q W_q (curl_i curl_j - div_i div_j)
+// over the module's own 6-point table (8-digit values, :181-184), with the
+// tangential surface curl and divergence of the 8 triangle basis functions
+// in the triangle frame.  With lambda_i the barycentric coordinates:
+//   edge (p, q):  curl = 3 (g_q x g_p) lambda_p and 3 (g_q x g_p) lambda_q,
+//                 div  = |g_p|^2 lambda_q - (g_p.g_q) lambda_p and
+//                        (g_p.g_q) lambda_q - |g_q|^2 lambda_p;
+//   faces: the expressions below (vertices 0, 1, 2 of the sorted triangle).
+__global__ void k_tri_abc2(int64_t ntri, const int32_t* __restrict__ tv, const double* __restrict__ xyz, int64_t V,
+                           double* __restrict__ mat) {
+    const int64_t it = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (it >= ntri) return;
+    double P[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const int v = tv[k * ntri + it];
+        P[k][0] = xyz[v];
+        P[k][1] = xyz[V + v];
+        P[k][2] = xyz[2 * V + v];
+    }
+    double e1[3], e2[3], zh[3], xh[3], yh[3];
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = P[1][d] - P[0][d];
+        e2[d] = P[2][d] - P[0][d];
+    }
+    zh[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    zh[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    zh[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    const double nz = sqrt(zh[0] * zh[0] + zh[1] * zh[1] + zh[2] * zh[2]);
+    const double nx = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    for (int d = 0; d < 3; ++d) {
+        zh[d] /= nz;
+        xh[d] = e1[d] / nx;
+    }
+    yh[0] = zh[1] * xh[2] - zh[2] * xh[1];
+    yh[1] = zh[2] * xh[0] - zh[0] * xh[2];
+    yh[2] = zh[0] * xh[1] - zh[1] * xh[0];
+    const double ny = sqrt(yh[0] * yh[0] + yh[1] * yh[1] + yh[2] * yh[2]);
+    double X[3], Y[3];
+    for (int k = 0; k < 3; ++k) {
+        const double dx = P[k][0] - P[0][0], dy = P[k][1] - P[0][1], dz = P[k][2] - P[0][2];
+        X[k] = xh[0] * dx + xh[1] * dy + xh[2] * dz;
+        Y[k] = (yh[0] * dx + yh[1] * dy + yh[2] * dz) / ny;
+    }
+    double D[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) D[i][j] = sqrt((X[i] - X[j]) * (X[i] - X[j]) + (Y[i] - Y[j]) * (Y[i] - Y[j]));
+    const double sA = 0.5 * ((X[0] - X[2]) * (Y[1] - Y[0]) - (X[0] - X[1]) * (Y[2] - Y[0]));
+    const double sg = sA > 0 ? 1.0 : (sA < 0 ? -1.0 : 0.0);
+    const double A = fabs(sA);
+    // barycentric lambda_i = a_i + b_i x + c_i y (coefficients over 2A)
+    double a[3], b[3], c[3];
+    a[0] = (X[1] * Y[2] - Y[1] * X[2]) * sg / (2 * A);
+    a[1] = (X[2] * Y[0] - Y[2] * X[0]) * sg / (2 * A);
+    a[2] = (X[0] * Y[1] - Y[0] * X[1]) * sg / (2 * A);
+    b[0] = (Y[1] - Y[2]) * sg / (2 * A);
+    b[1] = (Y[2] - Y[0]) * sg / (2 * A);
+    b[2] = (Y[0] - Y[1]) * sg / (2 * A);
+    c[0] = (X[2] - X[1]) * sg / (2 * A);
+    c[1] = (X[0] - X[2]) * sg / (2 * A);
+    c[2] = (X[1] - X[0]) * sg / (2 * A);
+    const double W[6] = {0.22338159, 0.22338159, 0.22338159, 0.10995174, 0.10995174, 0.10995174};
+    const double Q1[6] = {0.10810302, 0.44594849, 0.44594849, 0.81684757, 0.09157621, 0.09157621};
+    const double Q2[6] = {0.44594849, 0.44594849, 0.10810302, 0.09157621, 0.09157621, 0.81684757};
+    const double Q3[6] = {0.44594849, 0.10810302, 0.44594849, 0.09157621, 0.81684757, 0.09157621};
+    const int ep[3] = {0, 1, 0}, eq[3] = {1, 2, 2};  // local edges (0,1), (1,2), (0,2)
+    double L[8];
+    for (int e = 0; e < 3; ++e) L[e] = L[e + 4] = D[ep[e]][eq[e]];
+    L[3] = D[0][2];
+    L[7] = D[0][1];
+    double M[8][8];
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) M[i][j] = 0.0;
+    for (int q = 0; q < 6; ++q) {
+        const double x = X[0] * Q1[q] + X[1] * Q2[q] + X[2] * Q3[q];
+        const double y = Y[0] * Q1[q] + Y[1] * Q2[q] + Y[2] * Q3[q];
+        double lam[3];
+        for (int k = 0; k < 3; ++k) lam[k] = a[k] + b[k] * x + c[k] * y;
+        double cu[8], dv[8];
+        for (int e = 0; e < 3; ++e) {
+            const int p0 = ep[e], p1 = eq[e];
+            const double cr = 3.0 * (b[p1] * c[p0] - b[p0] * c[p1]);
+            const double gpp = b[p0] * b[p0] + c[p0] * c[p0], gqq = b[p1] * b[p1] + c[p1] * c[p1];
+            const double gpq = b[p0] * b[p1] + c[p0] * c[p1];
+            cu[e] = cr * lam[p0];
+            cu[e + 4] = cr * lam[p1];
+            dv[e] = gpp * lam[p1] - gpq * lam[p0];
+            dv[e + 4] = gpq * lam[p1] - gqq * lam[p0];
+        }
+        cu[3] = -b[1] * (c[0] * lam[2] - c[2] * lam[0]) + c[1] * (b[0] * lam[2] - b[2] * lam[0]) +
+                2.0 * (b[0] * c[2] - b[2] * c[0]) * lam[1];
+        cu[7] = b[2] * (c[0] * lam[1] - c[1] * lam[0]) - c[2] * (b[0] * lam[1] - b[1] * lam[0]) -
+                2.0 * (b[0] * c[1] - b[1] * c[0]) * lam[2];
+        dv[3] = -b[1] * (b[0] * lam[2] - b[2] * lam[0]) - c[1] * (c[0] * lam[2] - c[2] * lam[0]);
+        dv[7] = b[2] * (b[0] * lam[1] - b[1] * lam[0]) + c[2] * (c[0] * lam[1] - c[1] * lam[0]);
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) M[i][j] += W[q] * (cu[i] * cu[j] - dv[i] * dv[j]);
+    }
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) mat[it * 64 + i * 8 + j] = L[i] * L[j] * M[i][j] * A;
+}
+
 // out[k] = sum_{j in [seg[k], seg[k+1])} src[idx[j]]  (fixed order)
 __global__ void k_seg_sum_real(int64_t n, const int32_t* __restrict__ seg, const int32_t* __restrict__ idx,
                                const double* __restrict__ src, double* __restrict__ out) {
@@ -334,11 +437,14 @@ extern "C" int emg_system_pattern_get(emg_ctx* c, int32_t* indptr, int32_t* indi
     return EMG_OK;
 }
 
-extern "C" int emg_bc_add(emg_ctx* c, int64_t n_tri, const int32_t* tri_vid, const double* uinc, int32_t* bc_id,
-                          int64_t* tri_offset) {
-    EMG_CHECK_CTX(c);
+namespace {
+// append the triangles of one surface term: kind 0 = Robin / port matrix and
+// forcing (k_tri_terms), kind 1 = second-order ABC correction (k_tri_abc2)
+int add_surface_tris(emg_ctx* c, int kind, int64_t n_tri, const int32_t* tri_vid, const double* uinc,
+                     int32_t* bc_id, int64_t* tri_offset) {
     if (c->n_nodes == 0) return fail(c, EMG_ERR_STATE, "emg_bc_add: no mesh");
     if (n_tri < 1 || !tri_vid) return fail(c, EMG_ERR_ARG, "emg_bc_add: empty triangle list");
+    if (kind == 1 && uinc) return fail(c, EMG_ERR_ARG, "emg_bc_add_abc2: no forcing for an ABC term");
     for (int64_t k = 0; k < 3 * n_tri; ++k)
         if (tri_vid[k] < 0 || tri_vid[k] >= c->n_nodes) return fail(c, EMG_ERR_ARG, "emg_bc_add: vertex id out of range");
     hipStream_t st = c->stream;
@@ -361,8 +467,11 @@ extern "C" int emg_bc_add(emg_ctx* c, int64_t n_tri, const int32_t* tri_vid, con
         EMG_HIP(c, du.alloc(18 * n_tri));
         EMG_HIP(c, hipMemcpyAsync(du.p, uinc, sizeof(double2) * 18 * n_tri, hipMemcpyHostToDevice, st));
     }
-    k_tri_terms<<<grid_for(n_tri, 64), 64, 0, st>>>(n_tri, dtv.p, c->xyz.p, c->n_nodes, uinc ? du.p : nullptr,
-                                                    mat.p + 64 * off, uinc ? force.p + 8 * off : nullptr);
+    if (kind == 0)
+        k_tri_terms<<<grid_for(n_tri, 64), 64, 0, st>>>(n_tri, dtv.p, c->xyz.p, c->n_nodes, uinc ? du.p : nullptr,
+                                                        mat.p + 64 * off, uinc ? force.p + 8 * off : nullptr);
+    else
+        k_tri_abc2<<<grid_for(n_tri, 64), 64, 0, st>>>(n_tri, dtv.p, c->xyz.p, c->n_nodes, mat.p + 64 * off);
     EMG_HIP(c, hipGetLastError());
     EMG_HIP(c, hipStreamSynchronize(st));
     std::swap(c->tri_mat.p, mat.p);
@@ -370,10 +479,24 @@ extern "C" int emg_bc_add(emg_ctx* c, int64_t n_tri, const int32_t* tri_vid, con
     std::swap(c->tri_force.p, force.p);
     std::swap(c->tri_force.n, force.n);
     c->n_tri_total = off + n_tri;
-    if (bc_id) *bc_id = c->n_bc++;
+    if (bc_id) *bc_id = c->n_bc;
+    c->n_bc++;
     if (tri_offset) *tri_offset = off;
     c->have_surface = false;
     return EMG_OK;
+}
+}  // namespace
+
+extern "C" int emg_bc_add(emg_ctx* c, int64_t n_tri, const int32_t* tri_vid, const double* uinc, int32_t* bc_id,
+                          int64_t* tri_offset) {
+    EMG_CHECK_CTX(c);
+    return add_surface_tris(c, 0, n_tri, tri_vid, uinc, bc_id, tri_offset);
+}
+
+extern "C" int emg_bc_add_abc2(emg_ctx* c, int64_t n_tri, const int32_t* tri_vid, int32_t* bc_id,
+                               int64_t* tri_offset) {
+    EMG_CHECK_CTX(c);
+    return add_surface_tris(c, 1, n_tri, tri_vid, nullptr, bc_id, tri_offset);
 }
 
 extern "C" int emg_tri_terms_get(emg_ctx* c, double* mat, double* force) {

@@ -137,26 +137,32 @@ class FrequencySweep:
         self.ctx.call("emg_system_reduce", ids32.shape[0], ptr(ids32))
         self.n_red = ids32.shape[0]
 
-        # boundary triangles of every Robin BC on the device
+        # surface terms on the device, one per Robin BC plus one per
+        # second-order ABC correction: (kind, bc); the solve scales term i by
+        # gamma_i(k0)
+        self.surface = [("robin", b) for b in self.robin]
+        self.surface += [("abc2", b) for b in self.robin if getattr(b, "order", 1) == 2]
         tri_off = []
-        self._port_index = {}
-        for ib, b in enumerate(self.robin):
+        for kind, b in self.surface:
             tv = np.ascontiguousarray(m.tris[:, b.tri_ids], dtype=np.int32)
-            u = None
-            if b.excited:
-                pts = dunavant_points(m.nodes, m.tris, b.tri_ids)  # (n,6,3)
-                shp = b.uinc_shape(pts[..., 0].reshape(-1), pts[..., 1].reshape(-1), pts[..., 2].reshape(-1))
-                u = np.ascontiguousarray(np.asarray(shp, dtype=np.complex128).reshape(3, -1).T)  # (n*6, 3)
             bid = ctypes.c_int32()
             off = ctypes.c_int64()
-            self.ctx.call("emg_bc_add", tv.shape[1], ptr(tv), ptr(u) if u is not None else None,
-                          ctypes.byref(bid), ctypes.byref(off))
+            if kind == "abc2":
+                self.ctx.call("emg_bc_add_abc2", tv.shape[1], ptr(tv), ctypes.byref(bid), ctypes.byref(off))
+            else:
+                u = None
+                if b.excited:
+                    pts = dunavant_points(m.nodes, m.tris, b.tri_ids)  # (n,6,3)
+                    shp = b.uinc_shape(pts[..., 0].reshape(-1), pts[..., 1].reshape(-1), pts[..., 2].reshape(-1))
+                    u = np.ascontiguousarray(np.asarray(shp, dtype=np.complex128).reshape(3, -1).T)  # (n*6, 3)
+                self.ctx.call("emg_bc_add", tv.shape[1], ptr(tv), ptr(u) if u is not None else None,
+                              ctypes.byref(bid), ctypes.byref(off))
             tri_off.append(off.value)
         self.tri_off = tri_off
 
-        # surface entries (row, col, bc) over reduced rows, contributions in tri order
+        # surface entries (row, col, term) over reduced rows, contributions in tri order
         rows_l, cols_l, bcs_l, con_l = [], [], [], []
-        for ib, b in enumerate(self.robin):
+        for ib, (kind, b) in enumerate(self.surface):
             t = b.tri_ids
             dof = f.tri_to_field[:, t]  # (8, n)
             rr = red[dof]
@@ -175,7 +181,7 @@ class FrequencySweep:
             r = c = bb = con = np.zeros(0, dtype=np.int64)
         order = np.lexsort((con, bb, c, r))
         r, c, bb, con = r[order], c[order], bb[order], con[order]
-        key = (r * (self.n_red + 1) + c) * (len(self.robin) + 1) + bb
+        key = (r * (self.n_red + 1) + c) * (len(self.surface) + 1) + bb
         head = np.ones(key.shape[0], dtype=bool)
         head[1:] = key[1:] != key[:-1]
         starts = np.nonzero(head)[0]
@@ -195,7 +201,7 @@ class FrequencySweep:
         # forcing vectors: one per port, rows p*n_red + r
         frows, fcon = [], []
         for ip, p in enumerate(self.ports):
-            ib = self.robin.index(p)
+            ib = self.robin.index(p)  # Robin terms come first in self.surface
             t = p.tri_ids
             rr = red[f.tri_to_field[:, t]].T  # (n, 8)
             g = tri_off[ib] + np.arange(t.shape[0])
@@ -309,14 +315,15 @@ class FrequencySweep:
         """k2, gamma [batch][bc] and rhs coefficients [batch][port] for the
         (frequency, active port) pairs in frequency-major order."""
         freqs = np.asarray(freqs, dtype=np.float64)
-        nb = len(self.robin)
+        nb = len(self.surface)
         npt = len(self.ports)
         k0 = 2 * np.pi * freqs / C0
         k2 = np.repeat(k0 ** 2, npt)
         gam = np.zeros((freqs.shape[0] * npt, max(nb, 1)), dtype=np.complex128)
         coef = np.zeros((freqs.shape[0] * npt, npt), dtype=np.complex128)
         for i, k in enumerate(k0):
-            g = np.array([b.get_gamma(k) for b in self.robin], dtype=np.complex128)
+            g = np.array([b.abc2_coefficient(k) if kind == "abc2" else b.get_gamma(k)
+                          for kind, b in self.surface], dtype=np.complex128)
             for j, p in enumerate(self.ports):
                 row = i * npt + j
                 gam[row, :nb] = g

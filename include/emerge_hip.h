@@ -122,6 +122,13 @@ int emg_system_pattern_get(emg_ctx* ctx, int32_t* indptr, int32_t* indices, int6
  * Triangles of all BCs are concatenated; *tri_offset is this BC's first. */
 int emg_bc_add(emg_ctx* ctx, int64_t n_tri, const int32_t* tri_vid, const double* uinc,
                int32_t* bc_id, int64_t* tri_offset);
+/* Add the boundary triangles of a second-order absorbing boundary as one
+ * more surface term: per triangle the 8x8 correction of _abc_order_2_terms
+ * without its factor cf (ASM/robin_abc_order2.py:188-311); the solve scales
+ * it by the gamma given for this term, 1j*c2/k0 (ASM/assembler.py:342-348).
+ * The same triangles also carry the order-2 Robin term (emg_bc_add with
+ * gamma = 1j*k0*c1). */
+int emg_bc_add_abc2(emg_ctx* ctx, int64_t n_tri, const int32_t* tri_vid, int32_t* bc_id, int64_t* tri_offset);
 /* Copy the per-triangle matrices (64 per triangle) and forcing vectors
  * (8 complex per triangle); either may be NULL. */
 int emg_tri_terms_get(emg_ctx* ctx, double* mat, double* force);

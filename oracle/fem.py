@@ -56,6 +56,12 @@ def assemble_freq_matrix(mesh, maps: dict, E, B, pec_tris, robin_bcs, frequency:
                 port_vectors[bc.port_number] += b_p
             else:
                 Bempty = robin.assemble_robin_bc(mesh.nodes, mesh.tris, Bempty, bc.tri_ids, gamma)
+            # second-order ABC correction (ASM/assembler.py:342-348)
+            if getattr(bc, "order", 1) == 2:
+                from . import abc2
+                c2 = bc.o2coeffs[bc.abctype][1]
+                Bempty = Bempty + abc2.abc_order_2_matrix(mesh.nodes, mesh.tris, mesh.n_tris, bc.tri_ids,
+                                                          1j * c2 / K0)
         from .topology import empty_tri_rowcol
         rows, cols = empty_tri_rowcol(maps["tri_to_field"])
         ids = np.argwhere(Bempty != 0)[:, 0]

@@ -202,3 +202,42 @@ def test_cavity_lumped_port_matches_superlu(precision):
         ref = o["fields"][1][ids]
         assert np.linalg.norm(x[ids] - ref) / np.linalg.norm(ref) <= 1e-6
     print(f"cavity S11={res.S[:, 0, 0]} iters={res.iters.ravel().tolist()}")
+
+
+def test_abc2_triangle_terms_match_oracle():
+    """Device second-order ABC correction per triangle (without cf) vs the
+    oracle restatement of _abc_order_2_terms."""
+    from emerge_amd._lib import ptr
+    from oracle import abc2
+    c = wg_case.abc_case(2)
+    sw = FrequencySweep(c["mesh"], c["basis"], c["problem"].eps_r, c["problem"].inv_mu_r, c["prod_bcs"])
+    kinds = [k for k, _ in sw.surface]
+    ia = kinds.index("abc2")
+    n_all = sum(b.tri_ids.shape[0] for _, b in sw.surface)
+    mat = np.empty(64 * n_all)
+    sw.ctx.call("emg_tri_terms_get", ptr(mat), None)
+    m = c["mesh"]
+    t = c["abc_tris"]
+    got = mat[64 * sw.tri_off[ia]: 64 * (sw.tri_off[ia] + t.shape[0])].reshape(-1, 8, 8)
+    for k, it in enumerate(t):
+        ref = abc2.abc_order_2_terms(m.nodes[:, m.tris[:, it]], 1.0).real
+        assert np.abs(got[k] - ref).max() <= 1e-12 * np.abs(ref).max(), it
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_waveguide_abc_termination_matches_superlu(order):
+    """WR-90 guide closed by an absorbing boundary (order 1 or 2, type B):
+    residual, field and S11 against SuperLU and the oracle's extraction."""
+    c = wg_case.abc_case(order)
+    sw = FrequencySweep(c["mesh"], c["basis"], c["problem"].eps_r, c["problem"].inv_mu_r, c["prod_bcs"])
+    freqs = np.array([9e9, 11e9])
+    res = sw.solve(freqs, precision="mixed", tol=1e-8, fblock=2, restart=20000, max_cycles=40,
+                   method="cocg", inner_rel32=1e-2)
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = wg_case.abc_oracle_solution(float(f), order)
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6, (f, res.S[i], o["S"])
+        ids = o["ids"]
+        ref = o["fields"][1][ids]
+        assert np.linalg.norm(res.fields[(i, 1)][ids] - ref) / np.linalg.norm(ref) <= 1e-6
+    print(f"abc order {order}: S11={res.S[:, 0, 0]} iters={res.iters.ravel().tolist()}")

@@ -53,3 +53,30 @@ def s_from_fields(fields: dict, freq: float, cells=(11, 5, 18), jitter=0.0):
     c = case(cells, jitter)
     k0 = 2 * np.pi * freq / fem.C0
     return fem.s_parameters(c["mesh"], c["basis"].tet_to_field, fields, c["o_ports"], k0)
+
+
+@functools.lru_cache(maxsize=4)
+def abc_case(order: int, cells=(11, 5, 18)):
+    """The guide with port 2 replaced by an absorbing boundary of the given
+    order (type B) on z = L."""
+    c = case(cells)
+    m = c["mesh"]
+    t1 = m.plane_tris(2, 0.0)
+    t2 = m.plane_tris(2, L)
+    pec = np.setdiff1d(m.boundary_tris(), np.concatenate([t1, t2]))
+    cs1 = ([1, 0, 0], [0, 1, 0], [0, 0, 1], [WR90_A / 2, WR90_B / 2, 0.0])
+    prod_bcs = [pbc.PEC(pec), pbc.RectangularWaveguide(t1, 1, pbc.CoordinateSystem(*cs1), (WR90_A, WR90_B)),
+                pbc.AbsorbingBoundary(t2, order=order, abctype="B")]
+    o_bcs = [oports.RectangularWaveguide(t1, 1, oports.CS(*cs1), (WR90_A, WR90_B)),
+             oports.AbsorbingBoundary(t2, order=order, abctype="B")]
+    return dict(c, pec=pec, prod_bcs=prod_bcs, o_bcs=o_bcs, abc_tris=t2)
+
+
+@functools.lru_cache(maxsize=8)
+def abc_oracle_solution(freq: float, order: int):
+    c = abc_case(order)
+    K, b, pv, ids = fem.assemble_freq_matrix(c["mesh"], c["maps"], c["E"], c["B"], c["pec"], c["o_bcs"], freq)
+    fields = fem.solve_all_ports(K, b, pv, ids)
+    k0 = 2 * np.pi * freq / fem.C0
+    S = fem.s_parameters(c["mesh"], c["basis"].tet_to_field, fields, [c["o_bcs"][0]], k0)
+    return dict(K=K, b=b, pv=pv, ids=ids, fields=fields, S=S, k0=k0)
