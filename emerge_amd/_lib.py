@@ -51,6 +51,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_port_norms_get": (_c_int, [_p, _p]),
     "emg_solver_config": (_c_int, [_p, _c_int, _c_int, ctypes.c_double]),
     "emg_port_points_set": (_c_int, [_p, ctypes.c_int32, _p, _p, _p, _p, _p]),
+    "emg_material_set_tensor": (_c_int, [_p, _p, _p]),
     "emg_bc_add_abc2": (_c_int, [_p, _i64, _p, ctypes.POINTER(ctypes.c_int32), _pi64]),
     "emg_csr_set": (_c_int, [_p, _i64, _p, _p, _p]),
     "emg_csr_solve": (_c_int, [_p, ctypes.c_int32, _p, _c_int, ctypes.c_double, _c_int, _c_int, _p, _p, _p]),
@@ -166,6 +167,14 @@ class Context:
         if e.shape != (self.n_tets,) or m.shape != (self.n_tets,):
             raise ValueError("material arrays must have shape (n_tets,)")
         self.call("emg_material_set", ptr(e), ptr(m))
+
+    def material_set_tensor(self, eps_r: np.ndarray, mu_r: np.ndarray) -> None:
+        """(3, 3, n_tets) complex tensors eps_r and mu_r (not inverted)."""
+        e = np.ascontiguousarray(eps_r, dtype=np.complex128)
+        m = np.ascontiguousarray(mu_r, dtype=np.complex128)
+        if e.shape != (3, 3, self.n_tets) or m.shape != (3, 3, self.n_tets):
+            raise ValueError("material tensors must have shape (3, 3, n_tets)")
+        self.call("emg_material_set_tensor", ptr(e), ptr(m))
 
     def assemble(self) -> None:
         self.call("emg_assemble")

@@ -17,12 +17,9 @@ from scipy.sparse import csr_matrix
 from ._lib import Context
 
 
-def isotropic_scalars(tensor: np.ndarray, name: str) -> np.ndarray:
-    """(3,3,T) tensor -> (T,) scalars; raises if any tet is anisotropic.
-
-    The GPU element kernel takes one complex scalar per tet for eps_r and
-    mu_r; anisotropic tensors are rejected rather than approximated.
-    """
+def isotropic_scalars(tensor: np.ndarray, name: str) -> np.ndarray | None:
+    """(3,3,T) tensor -> (T,) scalars when every tet is isotropic (one
+    complex scalar times the identity), else None."""
     t = np.asarray(tensor)
     if t.ndim != 3 or t.shape[:2] != (3, 3):
         raise ValueError(f"{name} must have shape (3, 3, T)")
@@ -30,8 +27,7 @@ def isotropic_scalars(tensor: np.ndarray, name: str) -> np.ndarray:
     off = np.ones((3, 3), dtype=bool)
     np.fill_diagonal(off, False)
     if np.any(t[off] != 0) or np.any(t[1, 1] != d) or np.any(t[2, 2] != d):
-        raise NotImplementedError(f"{name}: anisotropic material tensors are not supported by the HIP "
-                                  "element kernel")
+        return None
     return np.ascontiguousarray(d, dtype=np.complex128)
 
 
@@ -94,9 +90,15 @@ class HipAssembler:
         return self._dev_mesh
 
     def tet_mass_stiffness_matrices(self, field, er: np.ndarray, ur: np.ndarray):
-        """ASM/curlcurl.py:145-173: (E curl-curl, B mass) as scipy CSR."""
+        """ASM/curlcurl.py:145-173: (E curl-curl, B mass) as scipy CSR.
+        Isotropic media take the scalar kernel; any anisotropic tet (or PML
+        tensor) sends the whole mesh through the tensor kernel."""
         eps = isotropic_scalars(er, "er")
         mu = isotropic_scalars(ur, "ur")
         dm = self.device_mesh(field)
-        dm.assemble(eps, 1.0 / mu)  # matinv's diagonal path (E/mth/optimized.py:354-357)
+        if eps is not None and mu is not None:
+            dm.assemble(eps, 1.0 / mu)  # matinv's diagonal path (E/mth/optimized.py:354-357)
+        else:
+            dm.ctx.material_set_tensor(er, ur)
+            dm.ctx.assemble()
         return dm.csr()

@@ -174,6 +174,222 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
     }
 }
 
+
+// General 3x3 complex material tensors (anisotropic media, PML): the same
+// schedule and slot sums, with the material inside the Gram products,
+// h_ef = C_e . (mu^-1 C_f) and g_xy = grad_x . (eps grad_y), contracted with
+// the ordered tables.  Four passes: (curl-curl, mass) x (real, imaginary
+// part); each pass writes one component of the CSR values.  Tensors are
+// stored per component, t fastest ((3,3,T) as the reference's arrays).
+__global__ __launch_bounds__(kAsmThreads) void k_assemble_rows_tensor(
+    const Chunk* __restrict__ chunks, int64_t n_chunks, const int32_t* __restrict__ inc_tet,
+    const uint8_t* __restrict__ inc_u, const int4* __restrict__ inc_vid, const uint16_t* __restrict__ grp_slot,
+    const uint16_t* __restrict__ contrib, const double* __restrict__ xyz, int64_t V, int64_t T,
+    const double2* __restrict__ eps9, const double2* __restrict__ imu9, double* __restrict__ Ev,
+    double* __restrict__ Bv) {
+    __shared__ double stage[20][kAsmThreads];  // element row (one component) of every lane
+    __shared__ double fac[kAsmThreads];        // real row factor of every lane
+    __shared__ double carry[kAsmThreads];      // partial sum of a slot continued from the previous lane
+    __shared__ int closed[kAsmThreads];        // whether that slot ends in this lane's range
+
+    const int64_t per = (n_chunks + 7) / 8;
+    const int64_t b = blockIdx.x;
+    const int64_t ci = (b % 8) * per + b / 8;
+    if (ci >= n_chunks) return;
+    const Chunk ch = chunks[ci];
+    const int tid = threadIdx.x;
+    const bool active = tid < ch.ninc;
+
+    int u = 0, t = 0;
+    ned2::TetVecs G;
+    double kf = 0.0, mf = 0.0;
+    uint32_t cw[10];
+    int slot_first = 0;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) cw[r] = 0;
+    if (active) {
+        const int64_t inc = int64_t(ch.inc0) + tid;
+        const uint2* cp = reinterpret_cast<const uint2*>(contrib + inc * 20);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const uint2 v = cp[r];
+            cw[2 * r] = v.x;
+            cw[2 * r + 1] = v.y;
+        }
+        slot_first = grp_slot[inc];
+        t = inc_tet[inc];
+        u = inc_u[inc];
+        const int4 wv = inc_vid[inc];
+        const int w[4] = {wv.x, wv.y, wv.z, wv.w};
+        int rk[4];
+        ned2::ranks4(w, rk);
+        double X[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            X[k][0] = xyz[w[k]];
+            X[k][1] = xyz[V + w[k]];
+            X[k][2] = xyz[2 * V + w[k]];
+        }
+        ned2::tet_vectors(X, rk, G);
+        double su = 0.0;
+#pragma unroll
+        for (int k = 0; k < 20; ++k) su = (k == u) ? G.d[ned2::len_index(k)] : su;
+        kf = su * G.inv6v * G.inv6v * G.inv6v * (1.0 / 120.0);
+        mf = su * G.inv6v * (1.0 / 5040.0);
+    }
+    const bool cont = (cw[0] & kCodeCont) != 0;
+    const bool tail_open = active && (cw[9] & (uint32_t(kCodeLast) << 16)) == 0;
+
+#pragma unroll 1
+    for (int pass = 0; pass < 4; ++pass) {
+        const int mat = pass >> 1, part = pass & 1;
+        if (active) {
+            // this pass's component of the material tensor, then its Gram products
+            double M[3][3];
+            const double2* src = mat == 0 ? imu9 : eps9;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const double2 v = src[int64_t(k) * T + t];
+                M[k / 3][k % 3] = part == 0 ? v.x : v.y;
+            }
+            double hg[36];
+            if (mat == 0) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    const double w0 = M[0][0] * G.c[f][0] + M[0][1] * G.c[f][1] + M[0][2] * G.c[f][2];
+                    const double w1 = M[1][0] * G.c[f][0] + M[1][1] * G.c[f][1] + M[1][2] * G.c[f][2];
+                    const double w2 = M[2][0] * G.c[f][0] + M[2][1] * G.c[f][1] + M[2][2] * G.c[f][2];
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) hg[6 * e + f] = G.c[e][0] * w0 + G.c[e][1] * w1 + G.c[e][2] * w2;
+                }
+            } else {
+#pragma unroll
+                for (int y = 0; y < 4; ++y) {
+                    const double w0 = M[0][0] * G.gl[y][0] + M[0][1] * G.gl[y][1] + M[0][2] * G.gl[y][2];
+                    const double w1 = M[1][0] * G.gl[y][0] + M[1][1] * G.gl[y][1] + M[1][2] * G.gl[y][2];
+                    const double w2 = M[2][0] * G.gl[y][0] + M[2][1] * G.gl[y][1] + M[2][2] * G.gl[y][2];
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) hg[4 * x + y] = G.gl[x][0] * w0 + G.gl[x][1] * w1 + G.gl[x][2] * w2;
+                }
+#pragma unroll
+                for (int k = 16; k < 36; ++k) hg[k] = 0.0;
+            }
+            bool pending = true;
+            while (true) {
+                const uint64_t m = __ballot(pending);
+                if (m == 0) break;
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                const int u0 = __builtin_amdgcn_readfirstlane(__shfl(u, leader));
+                if (pending && u == u0) {
+                    // inputs marked as rewritten on every trip: the switch
+                    // cases stay inside this loop (see k_assemble_rows)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) asm volatile("" : "+v"(G.d[k]));
+                    if (mat == 0) {
+#pragma unroll
+                        for (int k = 0; k < 36; ++k) asm volatile("" : "+v"(hg[k]));
+                        ned2::stiff_row_o(u0, hg, G.d, &stage[0][tid], kAsmThreads);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) asm volatile("" : "+v"(hg[k]));
+                        ned2::mass_row_o(u0, hg, G.d, &stage[0][tid], kAsmThreads);
+                    }
+                    pending = false;
+                }
+            }
+            fac[tid] = mat == 0 ? kf : mf;
+        }
+        __syncthreads();
+        double* out = (mat == 0 ? Ev : Bv) + 2 * int64_t(ch.slot0) + part;  // component `part` of slot 0
+        int slot = slot_first;
+        bool first = true;
+        double acc = 0.0;
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < 20; ++r) {
+                const uint32_t code = (cw[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                const int ln = (code >> 5) & 0xff;
+                acc = __builtin_fma(fac[ln], stage[code & 31][ln], acc);
+                if (code & kCodeLast) {
+                    if (first && cont) {
+                        carry[tid] = acc;
+                        closed[tid] = 1;
+                    } else {
+                        out[2 * slot] = acc;
+                    }
+                    ++slot;
+                    acc = 0.0;
+                    first = false;
+                }
+            }
+            if (tail_open && first && cont) {
+                carry[tid] = acc;
+                closed[tid] = 0;
+            }
+        }
+        __syncthreads();
+        if (tail_open && !(first && cont)) {
+            for (int q = tid + 1; q < ch.ninc; ++q) {
+                acc += carry[q];
+                if (closed[q]) break;
+            }
+            out[2 * slot] = acc;
+        }
+    }
+}
+
+// 3x3 complex inverse per tet (matinv, E/mth/optimized.py:342-370: exact
+// reciprocals on a diagonal tensor, adjugate / determinant otherwise);
+// (3,3,T) component layout
+__global__ void k_matinv(int64_t T, const double2* __restrict__ in, double2* __restrict__ out) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    double2 M[3][3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) M[k / 3][k % 3] = in[int64_t(k) * T + t];
+    auto mul = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    auto sub = [](double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); };
+    auto add = [](double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); };
+    auto neg = [](double2 a) { return make_double2(-a.x, -a.y); };
+    auto inv = [](double2 a) {
+        const double m = a.x * a.x + a.y * a.y;
+        return make_double2(a.x / m, -a.y / m);
+    };
+    auto zero = [](double2 a) { return a.x == 0.0 && a.y == 0.0; };
+    double2 R[3][3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k / 3][k % 3] = make_double2(0.0, 0.0);
+    if (zero(M[0][1]) && zero(M[0][2]) && zero(M[1][0]) && zero(M[1][2]) && zero(M[2][0]) && zero(M[2][1])) {
+        R[0][0] = inv(M[0][0]);
+        R[1][1] = inv(M[1][1]);
+        R[2][2] = inv(M[2][2]);
+    } else {
+        // the reference's term order, left to right
+        const double2 det = sub(add(add(sub(sub(mul(mul(M[0][0], M[1][1]), M[2][2]), mul(mul(M[0][0], M[1][2]), M[2][1])),
+                                            mul(mul(M[0][1], M[1][0]), M[2][2])),
+                                        mul(mul(M[0][1], M[1][2]), M[2][0])),
+                                    mul(mul(M[0][2], M[1][0]), M[2][1])),
+                                mul(mul(M[0][2], M[1][1]), M[2][0]));
+        R[0][0] = sub(mul(M[1][1], M[2][2]), mul(M[1][2], M[2][1]));
+        R[0][1] = add(neg(mul(M[0][1], M[2][2])), mul(M[0][2], M[2][1]));
+        R[0][2] = sub(mul(M[0][1], M[1][2]), mul(M[0][2], M[1][1]));
+        R[1][0] = add(neg(mul(M[1][0], M[2][2])), mul(M[1][2], M[2][0]));
+        R[1][1] = sub(mul(M[0][0], M[2][2]), mul(M[0][2], M[2][0]));
+        R[1][2] = add(neg(mul(M[0][0], M[1][2])), mul(M[0][2], M[1][0]));
+        R[2][0] = sub(mul(M[1][0], M[2][1]), mul(M[1][1], M[2][0]));
+        R[2][1] = add(neg(mul(M[0][0], M[2][1])), mul(M[0][1], M[2][0]));
+        R[2][2] = sub(mul(M[0][0], M[1][1]), mul(M[0][1], M[1][0]));
+        // out / det as numpy's complex division
+        const double dm = det.x * det.x + det.y * det.y;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const double2 a = R[k / 3][k % 3];
+            R[k / 3][k % 3] = make_double2((a.x * det.x + a.y * det.y) / dm, (a.y * det.x - a.x * det.y) / dm);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[int64_t(k) * T + t] = R[k / 3][k % 3];
+}
 }  // namespace
 
 extern "C" int emg_assemble(emg_ctx* c) {
@@ -183,6 +399,16 @@ extern "C" int emg_assemble(emg_ctx* c) {
     EMG_HIP(c, c->Ev.alloc(2 * c->nnz));
     EMG_HIP(c, c->Bv.alloc(2 * c->nnz));
     const int64_t per = (c->n_chunks + 7) / 8;
+    if (c->have_tensor) {
+        k_assemble_rows_tensor<<<unsigned(per * 8), kAsmThreads, 0, c->stream>>>(
+            c->chunks.p, c->n_chunks, c->inc_tet.p, c->inc_u.p, reinterpret_cast<const int4*>(c->inc_vid.p),
+            c->grp_slot.p, c->contrib.p, c->xyz.p, c->n_nodes, c->n_tets,
+            reinterpret_cast<const double2*>(c->eps9.p), reinterpret_cast<const double2*>(c->imu9.p), c->Ev.p,
+            c->Bv.p);
+        EMG_HIP(c, hipGetLastError());
+        c->have_values = true;
+        return EMG_OK;
+    }
     k_assemble_rows<<<unsigned(per * 8), kAsmThreads, 0, c->stream>>>(
         c->chunks.p, c->n_chunks, c->inc_tet.p, c->inc_u.p, reinterpret_cast<const int4*>(c->inc_vid.p),
         c->grp_slot.p, c->contrib.p, c->xyz.p, c->n_nodes, reinterpret_cast<const double2*>(c->eps.p),
@@ -190,5 +416,27 @@ extern "C" int emg_assemble(emg_ctx* c) {
         reinterpret_cast<double2*>(c->Ev.p), reinterpret_cast<double2*>(c->Bv.p));
     EMG_HIP(c, hipGetLastError());
     c->have_values = true;
+    return EMG_OK;
+}
+
+extern "C" int emg_material_set_tensor(emg_ctx* c, const double* eps_r, const double* mu_r) {
+    EMG_CHECK_CTX(c);
+    if (c->n_tets == 0) return fail(c, EMG_ERR_STATE, "emg_material_set_tensor: no mesh");
+    if (!eps_r || !mu_r) return fail(c, EMG_ERR_ARG, "emg_material_set_tensor: null tensor");
+    const int64_t T = c->n_tets;
+    hipStream_t s = c->stream;
+    DBuf<double> mu;
+    EMG_HIP(c, c->eps9.alloc(18 * T));
+    EMG_HIP(c, c->imu9.alloc(18 * T));
+    EMG_HIP(c, mu.alloc(18 * T));
+    EMG_HIP(c, hipMemcpyAsync(c->eps9.p, eps_r, sizeof(double) * 18 * T, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(mu.p, mu_r, sizeof(double) * 18 * T, hipMemcpyHostToDevice, s));
+    k_matinv<<<unsigned((T + 255) / 256), 256, 0, s>>>(T, reinterpret_cast<const double2*>(mu.p),
+                                                        reinterpret_cast<double2*>(c->imu9.p));
+    EMG_HIP(c, hipGetLastError());
+    EMG_HIP(c, hipStreamSynchronize(s));
+    c->have_material = true;
+    c->have_tensor = true;
+    c->have_values = false;
     return EMG_OK;
 }

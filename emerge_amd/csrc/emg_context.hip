@@ -107,6 +107,7 @@ int emg_material_set(emg_ctx* c, const double* eps, const double* inu) {
     EMG_HIP(c, hipMemcpyAsync(c->inu.p, inu, sizeof(double) * 2 * c->n_tets, hipMemcpyHostToDevice, c->stream));
     EMG_HIP(c, hipStreamSynchronize(c->stream));
     c->have_material = true;
+    c->have_tensor = false;
     return EMG_OK;
 }
 

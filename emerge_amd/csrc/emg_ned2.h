@@ -132,6 +132,53 @@ __host__ __device__ constexpr int tk(int u, int v, int k) {
     return acc;
 }
 
+// Ordered tables for general (not necessarily symmetric) material tensors:
+// the mass term of (row letter x, column letter y) multiplies
+// grad lam_x . (eps grad lam_y), index 4x + y; the curl-curl term of (row
+// cross e, column cross f) multiplies C_e . (mu^-1 C_f), index 6e + f
+// (ASM/curlcurl.py:213-407: dot(G_row, Mm G_col), dot(C_row, Ms C_col)).
+// The reference fills the face-row x edge-column blocks by transposing the
+// edge-row x face-column ones (ASM/curlcurl.py:321-328), so there entry
+// (u, v) is entry (v, u) with the products taken in the (v, u) order.
+__host__ __device__ constexpr bool is_face(int u) { return (u >= 6 && u < 10) || u >= 16; }
+__host__ __device__ constexpr int tmo_direct(int u, int v, int k) {
+    const Dof A = dof(u), B = dof(v);
+    int acc = 0;
+    if (4 * A.r + B.r == k) acc += mono4(A.p, A.q, B.p, B.q);
+    if (4 * A.r + B.q == k) acc -= mono4(A.p, A.q, B.p, B.r);
+    if (4 * A.q + B.r == k) acc -= mono4(A.p, A.r, B.p, B.q);
+    if (4 * A.q + B.q == k) acc += mono4(A.p, A.r, B.p, B.r);
+    if (v >= 6 && v < 10 && ((u >= 6 && u < 10) || u >= 16)) {
+        const int fi = (u < 10) ? u - 6 : u - 16;
+        const int fj = v - 6;
+        const int a = kFace[fi][0], b = kFace[fi][1], e = kFace[fi][2];
+        const int c = kFace[fj][0], d = kFace[fj][1], f = kFace[fj][2];
+        const int dv = mono4(b, c, d, f) - mono4(b, c, d, e);
+        if (4 * a + f == k) acc += (u < 10) ? -dv : dv;
+    }
+    return acc;
+}
+__host__ __device__ constexpr int tmo(int u, int v, int k) {
+    return (is_face(u) && !is_face(v)) ? tmo_direct(v, u, k) : tmo_direct(u, v, k);
+}
+__host__ __device__ constexpr int tko_direct(int u, int v, int k) {
+    const Dof A = dof(u), B = dof(v);
+    int acc = 0;
+    for (int m = 0; m < 3; ++m) {
+        const CurlTerm a = curl_term(A, m);
+        if (a.coef == 0) continue;
+        for (int n = 0; n < 3; ++n) {
+            const CurlTerm b = curl_term(B, n);
+            if (b.coef == 0) continue;
+            if (6 * a.e + b.e == k) acc += a.coef * b.coef * mono2(a.lam, b.lam);
+        }
+    }
+    return acc;
+}
+__host__ __device__ constexpr int tko(int u, int v, int k) {
+    return (is_face(u) && !is_face(v)) ? tko_direct(v, u, k) : tko_direct(u, v, k);
+}
+
 template <int C>
 __device__ __forceinline__ double madd(double acc, double x) {
     if constexpr (C == 0) {
@@ -176,6 +223,29 @@ __device__ __forceinline__ void mass_row_impl(const double* g, const double* d, 
     ((mr[V * stride] = d[len_index(V)] * contract_m<U, V>(g, std::make_integer_sequence<int, 10>{})), ...);
 }
 
+template <int U, int V, int... K>
+__device__ __forceinline__ double contract_ko(const double* h, std::integer_sequence<int, K...>) {
+    double acc = 0.0;
+    ((acc = madd<tko(U, V, K)>(acc, h[K])), ...);
+    return acc;
+}
+template <int U, int V, int... K>
+__device__ __forceinline__ double contract_mo(const double* g, std::integer_sequence<int, K...>) {
+    double acc = 0.0;
+    ((acc = madd<tmo(U, V, K)>(acc, g[K])), ...);
+    return acc;
+}
+template <int U, int... V>
+__device__ __forceinline__ void stiff_row_o_impl(const double* h, const double* d, double* kr, int stride,
+                                                 std::integer_sequence<int, V...>) {
+    ((kr[V * stride] = d[len_index(V)] * contract_ko<U, V>(h, std::make_integer_sequence<int, 36>{})), ...);
+}
+template <int U, int... V>
+__device__ __forceinline__ void mass_row_o_impl(const double* g, const double* d, double* mr, int stride,
+                                                std::integer_sequence<int, V...>) {
+    ((mr[V * stride] = d[len_index(V)] * contract_mo<U, V>(g, std::make_integer_sequence<int, 16>{})), ...);
+}
+
 // Dispatch on a wave-uniform canonical row index.
 #define EMG_ROW_SWITCH(CALL)                                                            \
     switch (u) {                                                                        \
@@ -200,6 +270,17 @@ __device__ __forceinline__ void mass_row(int u, const double* g, const double* d
 #undef EMG_M_CALL
 }
 
+__device__ __forceinline__ void stiff_row_o(int u, const double* h, const double* d, double* kr, int stride) {
+#define EMG_K_CALL(n) stiff_row_o_impl<n>(h, d, kr, stride, std::make_integer_sequence<int, 20>{})
+    EMG_ROW_SWITCH(EMG_K_CALL)
+#undef EMG_K_CALL
+}
+__device__ __forceinline__ void mass_row_o(int u, const double* g, const double* d, double* mr, int stride) {
+#define EMG_M_CALL(n) mass_row_o_impl<n>(g, d, mr, stride, std::make_integer_sequence<int, 20>{})
+    EMG_ROW_SWITCH(EMG_M_CALL)
+#undef EMG_M_CALL
+}
+
 // Per-tet canonical geometry: Gram vectors and DOF lengths.
 struct TetGeom {
     double g[10];
@@ -214,11 +295,20 @@ __device__ __forceinline__ void ranks4(const int* w, int* rk) {
     for (int k = 0; k < 4; ++k) rk[k] = (w[0] < w[k]) + (w[1] < w[k]) + (w[2] < w[k]) + (w[3] < w[k]);
 }
 
+// Canonical gradient vectors gl (cofactors, 6V grad lam) and their pairwise
+// crosses c, for material tensors (the Gram products then carry the tensor).
+struct TetVecs {
+    double gl[4][3];
+    double c[6][3];
+    double d[6];
+    double inv6v;
+};
+
 // XL[k] = coordinates of tet-local vertex k (the mesh's own vertex order),
 // rk[k] = rank of its global id.  Cofactors and volume use the reference's
 // polynomials in local order (ASM/curlcurl.py:111-136) so that they round
 // exactly as the reference does, then move to the canonical frame.
-__device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int (&rk)[4], TetGeom& G) {
+__device__ __forceinline__ void tet_vectors(const double (&XL)[4][3], const int (&rk)[4], TetVecs& T) {
     // The 24-term volume and the cofactors cancel heavily for small tets far
     // from the origin; evaluate them with plain IEEE multiply/add (no FMA
     // contraction) so they round exactly like the reference.
@@ -243,7 +333,7 @@ __device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int
                             + x2*y1*z3/6 - x2*y1*z4/6 - x2*y3*z1/6 + x2*y3*z4/6 + x2*y4*z1/6 - x2*y4*z3/6
                             - x3*y1*z2/6 + x3*y1*z4/6 + x3*y2*z1/6 - x3*y2*z4/6 - x3*y4*z1/6 + x3*y4*z2/6
                             + x4*y1*z2/6 - x4*y1*z3/6 - x4*y2*z1/6 + x4*y2*z3/6 + x4*y3*z1/6 - x4*y3*z2/6);
-    G.inv6v = 1.0 / (6.0 * vol);
+    T.inv6v = 1.0 / (6.0 * vol);
     // canonical frame: gl[c] / X[c] belong to the local vertex of rank c
     double gl[4][3], X[4][3];
 #pragma unroll
@@ -260,27 +350,37 @@ __device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int
             X[c][d] = xv;
         }
 #pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int a = kEdge[e][0], b = kEdge[e][1];
+        T.c[e][0] = gl[a][1] * gl[b][2] - gl[a][2] * gl[b][1];
+        T.c[e][1] = gl[a][2] * gl[b][0] - gl[a][0] * gl[b][2];
+        T.c[e][2] = gl[a][0] * gl[b][1] - gl[a][1] * gl[b][0];
+        const double dx = X[a][0] - X[b][0], dy = X[a][1] - X[b][1], dz = X[a][2] - X[b][2];
+        T.d[e] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) T.gl[k][dd] = gl[k][dd];
+}
+
+__device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int (&rk)[4], TetGeom& G) {
+#pragma clang fp contract(off)
+    TetVecs T;
+    tet_vectors(XL, rk, T);
+    G.inv6v = T.inv6v;
+#pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = a; b < 4; ++b)
-            G.g[gram_index(a, b)] = gl[a][0] * gl[b][0] + gl[a][1] * gl[b][1] + gl[a][2] * gl[b][2];
-    double c[6][3], d[6];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) {
-        const int a = kEdge[e][0], b = kEdge[e][1];
-        c[e][0] = gl[a][1] * gl[b][2] - gl[a][2] * gl[b][1];
-        c[e][1] = gl[a][2] * gl[b][0] - gl[a][0] * gl[b][2];
-        c[e][2] = gl[a][0] * gl[b][1] - gl[a][1] * gl[b][0];
-        const double dx = X[a][0] - X[b][0], dy = X[a][1] - X[b][1], dz = X[a][2] - X[b][2];
-        d[e] = sqrt(dx * dx + dy * dy + dz * dz);
-    }
+            G.g[gram_index(a, b)] = T.gl[a][0] * T.gl[b][0] + T.gl[a][1] * T.gl[b][1] + T.gl[a][2] * T.gl[b][2];
 #pragma unroll
     for (int e = 0; e < 6; ++e)
 #pragma unroll
         for (int f = e; f < 6; ++f)
-            G.h[cross_gram_index(e, f)] = c[e][0] * c[f][0] + c[e][1] * c[f][1] + c[e][2] * c[f][2];
+            G.h[cross_gram_index(e, f)] = T.c[e][0] * T.c[f][0] + T.c[e][1] * T.c[f][1] + T.c[e][2] * T.c[f][2];
 #pragma unroll
-    for (int e = 0; e < 6; ++e) G.d[e] = d[e];
+    for (int e = 0; e < 6; ++e) G.d[e] = T.d[e];
 }
 
 }  // namespace ned2

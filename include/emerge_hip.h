@@ -89,6 +89,16 @@ int emg_pattern_stats(emg_ctx* ctx, int64_t* n_chunks, int64_t* n_incidences,
  * ASM/assembler.py:240) and inv_mu_r = 1/mu_r (E/mth/optimized.py:354-357). */
 int emg_material_set(emg_ctx* ctx, const double* eps_r, const double* inv_mu_r);
 
+/* General 3x3 complex material tensors per tet, complex128 interleaved in
+ * the reference's (3,3,T) layout (component (i,j) of tet t at
+ * 2*((3i+j)*T + t)): eps_r (after the loss update of ASM/assembler.py:240)
+ * and mu_r; the device inverts mu_r as matinv (E/mth/optimized.py:342-370).
+ * emg_assemble then evaluates dot(G_row, eps G_col) and
+ * dot(C_row, mu^-1 C_col) as ned2_tet_stiff_mass (ASM/curlcurl.py:180-412)
+ * does for anisotropic media and PML.  emg_material_set returns to the
+ * isotropic path. */
+int emg_material_set_tensor(emg_ctx* ctx, const double* eps_r, const double* mu_r);
+
 /* Assemble E (curl-curl) and B (mass) CSR values on the device.
  * Replaces tet_mass_stiffness_matrices / _matrix_builder /
  * ned2_tet_stiff_mass (ASM/curlcurl.py:145-173, 419-466, 180-412) plus the

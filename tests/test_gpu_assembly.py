@@ -98,8 +98,48 @@ def test_hip_assembler_dropin_interface():
     assert _rowwise_rel_err(E, Eo) <= TOL and _rowwise_rel_err(B, Bo) <= TOL
     aniso = cc.isotropic(eps)
     aniso[0, 1] = 0.1
-    with pytest.raises(NotImplementedError):
-        asm.tet_mass_stiffness_matrices(f, aniso, cc.isotropic(mu))
+    E, B = asm.tet_mass_stiffness_matrices(f, aniso, cc.isotropic(mu))
+    Eo, Bo = cc.tet_mass_stiffness_matrices(m, f.tet_to_field, aniso, cc.isotropic(mu))
+    assert _rowwise_rel_err(E, Eo) <= TOL and _rowwise_rel_err(B, Bo) <= TOL
+
+
+def _random_tensors(T, seed, symmetric=False):
+    rng = np.random.default_rng(seed)
+    def one(scale):
+        A = rng.uniform(-0.3, 0.3, (3, 3, T)) + 1j * rng.uniform(-0.05, 0.05, (3, 3, T))
+        if symmetric:
+            A = (A + A.transpose(1, 0, 2)) / 2
+        for k in range(3):
+            A[k, k] += rng.uniform(1.0, scale, T) * (1 - 1j * rng.uniform(0, 0.02, T))
+        return A.astype(np.complex128)
+    return one(10.0), one(2.0)
+
+
+@pytest.mark.parametrize("symmetric,seed", [(True, 3), (False, 4)])
+def test_tensor_assembly_matches_oracle(hip_ctx, symmetric, seed):
+    """General 3x3 complex eps_r and mu_r per tet (PML-like diagonal blocks
+    plus off-diagonal coupling; non-symmetric when symmetric=False)."""
+    m, f, _, _ = _problem((4, 3, 3), 0.15, seed, shuffle=True)
+    er, ur = _random_tensors(m.n_tets, seed, symmetric)
+    dm = MeshOnDevice(hip_ctx, m.nodes, m.tets, f.tet_to_field, f.n_field)
+    hip_ctx.material_set_tensor(er, ur)
+    hip_ctx.assemble()
+    E, B = dm.csr()
+    Eo, Bo = cc.tet_mass_stiffness_matrices(m, f.tet_to_field, er, ur)
+    np.testing.assert_array_equal(E.indices, Eo.indices)
+    assert _rowwise_rel_err(E, Eo) <= TOL
+    assert _rowwise_rel_err(B, Bo) <= TOL
+
+
+def test_tensor_path_equals_scalar_path_on_isotropic_media(hip_ctx):
+    m, f, eps, mu = _problem((5, 4, 3), 0.15, 8)
+    dm = MeshOnDevice(hip_ctx, m.nodes, m.tets, f.tet_to_field, f.n_field)
+    dm.assemble(eps, 1.0 / mu)
+    E1, B1 = dm.csr()
+    hip_ctx.material_set_tensor(cc.isotropic(eps), cc.isotropic(mu))
+    hip_ctx.assemble()
+    E2, B2 = dm.csr()
+    assert _rowwise_rel_err(E2, E1) <= 1e-13 and _rowwise_rel_err(B2, B1) <= 1e-13
 
 
 def sampled_rows_oracle(m, f, eps, mu, rows):
