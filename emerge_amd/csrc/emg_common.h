@@ -136,6 +136,16 @@ struct emg_ctx {
     int idr_s = 4;
     double inner_rel32 = 1e-5;    // FP32 inner reduction per refinement step
 
+    // auxiliary gradient space of the COCG preconditioner (emg_aux_set)
+    int64_t n_escalated = 0;      // systems of the last emg_sweep_solve finished with FP64 inner solves
+    int64_t aux_n = 0;            // nodal functions; 0 = Jacobi only
+    int aux_nsurf = 0;
+    emg::DBuf<int32_t> aux_gptr, aux_gidx, aux_tptr, aux_tidx;
+    emg::DBuf<double> aux_gval64, aux_tval64;
+    emg::DBuf<float> aux_gval32, aux_tval32;
+    emg::DBuf<double> aux_dvol;   // 2 x aux_n
+    emg::DBuf<double> aux_dsurf;  // aux_nsurf x aux_n
+
     // generic CSR system handed in by the Solver plug-in
     int64_t g_n = 0, g_nnz = 0;
     emg::DBuf<int32_t> g_ptr, g_idx, g_diag;

@@ -40,6 +40,19 @@ struct Op {
     const int32_t* sbc = nullptr;
     const R* sval = nullptr;
     const int32_t* diag = nullptr; // slot of the diagonal per row (-1 if none)
+    // auxiliary gradient space of the COCG preconditioner D^-1 + G D_g^-1 G^T
+    // (aux_n nodal functions; 0 = Jacobi only): G (n x aux_n) and G^T as CSR,
+    // D_g = -k2 dvol + sum_s gam[s] dsurf[s] per nodal function
+    int64_t aux_n = 0;
+    int aux_nsurf = 0;
+    const int32_t* gptr = nullptr;
+    const int32_t* gidx = nullptr;
+    const R* gval = nullptr;
+    const int32_t* tptr = nullptr;
+    const int32_t* tidx = nullptr;
+    const R* tval = nullptr;
+    const double2* dvol = nullptr;
+    const double* dsurf = nullptr;  // [aux_nsurf][aux_n]
 };
 
 inline unsigned grid_for(int64_t n, int block) { return unsigned((n + block - 1) / block); }
@@ -1012,19 +1025,20 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
     }
 }
 
-// w = A_s z; then per row q = w + beta q and p = z + beta p (so that
-// q = A_s p without gathering p), and per-workgroup partial sums of p^T q.
+// q = A_s p and per-workgroup partial sums of p^T q.  q is formed from p
+// every iteration (not by the recurrence q = A z + beta q, whose drift from
+// A p breaks complex64 COCG on strongly indefinite systems).
 // One wavefront per row; a lane covers V consecutive systems of the batch
 // (V/2 16-byte gathers per matrix entry) and every G-th entry of the row,
 // two entries per step so that their gathers overlap.  The butterfly
 // reduction leaves each row sum in all G lanes of its systems, so the row
-// end (surface terms, q/p update, p^T q) is spread one system per lane, and
-// those lanes load their q, p, z before the gathers.
+// end (surface terms, q store, p^T q) is spread one system per lane, and
+// those lanes load their p before the gathers.
 template <typename R, int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
                                                         const cplx<R>* __restrict__ gam,
-                                                        const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p,
-                                                        cplx<R>* __restrict__ q, CgState st) {
+                                                        const cplx<R>* __restrict__ p, cplx<R>* __restrict__ q,
+                                                        CgState st) {
     constexpr int VM = sizeof(R) == 4 ? 8 : 4;  // 8 complex64 or 4 complex128: 64 bytes per lane
     constexpr int V = F >= VM ? VM : F;         // systems per lane
     constexpr int FG = F / V;                   // lanes per entry
@@ -1038,7 +1052,6 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     R kk[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) kk[v] = op.B ? k2[f0 + v] : R(0);
-    const cplx<R> bf = from_d<R>(st.beta[fo]);
     double2 dot = make_double2(0.0, 0.0);
     // XCD-aware rows: workgroups are dealt round-robin over the 8 XCDs;
     // XCD x walks rows [x n/8, (x+1) n/8) so that its L2 holds the part of
@@ -1049,15 +1062,11 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     const int64_t nw = int64_t(gridDim.x / 8) * W;
     for (int64_t row = r0 + int64_t(blockIdx.x / 8) * W + wv; row < r1; row += nw) {
         const int64_t o = row * F + fo;
-        R qx = R(0), qy = R(0), px = R(0), py = R(0), zx = R(0), zy = R(0);
+        R px = R(0), py = R(0);
         if (own) {
-            const cplx<R> qv = q[o], pv = p[o], zv = z[o];
-            qx = qv.x;
-            qy = qv.y;
+            const cplx<R> pv = p[o];
             px = pv.x;
             py = pv.y;
-            zx = zv.x;
-            zy = zv.y;
         }
         R ar[V], ai[V], br[V], bi[V];
 #pragma unroll
@@ -1065,10 +1074,10 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
         const int j1 = op.ptr[row + 1];
         int j = op.ptr[row] + g;
         for (; j + G < j1; j += 2 * G) {
-            cg_entry<R, F, V>(op, z, kk, f0, j, ar, ai);
-            cg_entry<R, F, V>(op, z, kk, f0, j + G, br, bi);
+            cg_entry<R, F, V>(op, p, kk, f0, j, ar, ai);
+            cg_entry<R, F, V>(op, p, kk, f0, j + G, br, bi);
         }
-        if (j < j1) cg_entry<R, F, V>(op, z, kk, f0, j, ar, ai);
+        if (j < j1) cg_entry<R, F, V>(op, p, kk, f0, j, ar, ai);
         // the pick is done on scalars: a select of vector-type structs is
         // lowered to an indexed stack load for complex128
         R ax = R(0), ay = R(0);
@@ -1089,14 +1098,11 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
                 for (int k = op.sptr[row]; k < op.sptr[row + 1]; ++k) {
                     const cplx<R> gv = gam[op.sbc[k] * F + fo];
                     const R sv = op.sval[k];
-                    a = cfma(cmk<R>(gv.x * sv, gv.y * sv), z[int64_t(op.scol[k]) * F + fo], a);
+                    a = cfma(cmk<R>(gv.x * sv, gv.y * sv), p[int64_t(op.scol[k]) * F + fo], a);
                 }
             }
-            const cplx<R> qn = cfma(bf, cmk<R>(qx, qy), a);
-            const cplx<R> pn = cfma(bf, cmk<R>(px, py), cmk<R>(zx, zy));
-            q[o] = qn;
-            p[o] = pn;
-            const double2 t = to_d(cmul(pn, qn));
+            q[o] = a;
+            const double2 t = to_d(cmul(cmk<R>(px, py), a));
             dot.x += t.x;
             dot.y += t.y;
         }
@@ -1131,6 +1137,15 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_alpha(int nblk, CgState st) {
     }
 }
 
+// p = z + beta p (beta = 0 on the first iteration of an inner solve)
+template <typename R, int F>
+__global__ void k_cg_pupd(int64_t nq, const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p, CgState st) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const cplx<R> b = from_d<R>(st.beta[k % F]);
+    p[k] = cfma(b, p[k], z[k]);
+}
+
 // dx += alpha p; r -= alpha q; z = dinv o r; per-workgroup partial sums of
 // r^T z and ||r||^2.  init: only z and the sums of the starting residual
 // (dx, p and q zeroed).
@@ -1138,7 +1153,7 @@ template <typename R, int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_update(int64_t nq, cplx<R>* __restrict__ p, cplx<R>* __restrict__ q,
                                                           const cplx<R>* __restrict__ dinv, cplx<R>* __restrict__ r,
                                                           cplx<R>* __restrict__ z, cplx<R>* __restrict__ dx, CgState st,
-                                                          int init) {
+                                                          int init, int aux) {
     __shared__ double2 red[kCgThreads];
     const int tid = threadIdx.x;
     const int f = tid % F;  // blockDim and the grid stride are multiples of F
@@ -1158,11 +1173,14 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(int64_t nq, cplx<R>* _
             rv.y -= aq.y;
             r[k] = rv;
         }
-        const cplx<R> zv = cmul(dinv[k], rv);
-        z[k] = zv;
-        const double2 t = to_d(cmul(rv, zv));
-        rz.x += t.x;
-        rz.y += t.y;
+        if (!aux) {
+            // with the auxiliary space, z and r^T z come from k_aux_z
+            const cplx<R> zv = cmul(dinv[k], rv);
+            z[k] = zv;
+            const double2 t = to_d(cmul(rv, zv));
+            rz.x += t.x;
+            rz.y += t.y;
+        }
         rr += cabs2(rv);
     }
     // fixed-order workgroup sums per f: threads f, f+F, f+2F, ...
@@ -1176,7 +1194,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(int64_t nq, cplx<R>* _
         }
         __syncthreads();
     }
-    if (tid < F) st.part[int64_t(blockIdx.x) * 2 * F + tid] = red[tid];
+    if (tid < F && !aux) st.part[int64_t(blockIdx.x) * 2 * F + tid] = red[tid];
     __syncthreads();
     red[tid] = make_double2(rr, 0.0);
     __syncthreads();
@@ -1240,6 +1258,93 @@ __global__ void k_cg_accum(int64_t nq, const cplx<R>* __restrict__ dx, const dou
     x[k].y += sc * d.y;
 }
 
+// D_g^-1 per nodal function and system: 1 / (-k2[f] dvol[j] + sum_s gam[s][f] dsurf[s][j])
+template <typename R, int F>
+__global__ void k_aux_dg(Op<R> op, const double* __restrict__ k2, const double2* __restrict__ gam,
+                         cplx<R>* __restrict__ dginv) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= op.aux_n * F) return;
+    const int64_t j = k / F;
+    const int f = int(k % F);
+    const double2 dv = op.dvol[j];
+    double dr = -k2[f] * dv.x, di = -k2[f] * dv.y;
+    for (int sb = 0; sb < op.aux_nsurf; ++sb) {
+        const double w = op.dsurf[int64_t(sb) * op.aux_n + j];
+        dr += gam[sb * F + f].x * w;
+        di += gam[sb * F + f].y * w;
+    }
+    const double m2 = dr * dr + di * di;
+    dginv[k] = m2 > 0 ? from_d<R>(make_double2(dr / m2, -di / m2)) : cmk<R>(R(0), R(0));
+}
+
+// t = D_g^-1 o (G^T r), layout [nodal][f]
+template <typename R, int F>
+__global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restrict__ r,
+                                                const cplx<R>* __restrict__ dginv, cplx<R>* __restrict__ t) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= op.aux_n * F) return;
+    const int64_t j = k / F;
+    const int f = int(k % F);
+    R ax = R(0), ay = R(0);
+    for (int e = op.tptr[j]; e < op.tptr[j + 1]; ++e) {
+        const R w = op.tval[e];
+        const cplx<R> v = r[int64_t(op.tidx[e]) * F + f];
+        ax += w * v.x;
+        ay += w * v.y;
+    }
+    t[k] = cmul(dginv[k], cmk<R>(ax, ay));
+}
+
+// z = dinv o r + G t and per-workgroup partial sums of r^T z (same grid,
+// stride and partial layout as k_cg_update, which leaves the ||r||^2 half)
+template <typename R, int F>
+__global__ __launch_bounds__(kCgThreads) void k_aux_z(int64_t nq, Op<R> op, const cplx<R>* __restrict__ dinv,
+                                                      const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ t,
+                                                      cplx<R>* __restrict__ z, CgState st) {
+    __shared__ double2 red[kCgThreads];
+    const int tid = threadIdx.x;
+    double2 rz = make_double2(0.0, 0.0);
+    for (int64_t k = int64_t(blockIdx.x) * kCgThreads + tid; k < nq; k += int64_t(gridDim.x) * kCgThreads) {
+        const int64_t row = k / F;
+        const int f = int(k % F);
+        const cplx<R> rv = r[k];
+        cplx<R> zv = cmul(dinv[k], rv);
+        for (int e = op.gptr[row]; e < op.gptr[row + 1]; ++e) {
+            const R w = op.gval[e];
+            const cplx<R> v = t[int64_t(op.gidx[e]) * F + f];
+            zv.x += w * v.x;
+            zv.y += w * v.y;
+        }
+        z[k] = zv;
+        const double2 d = to_d(cmul(rv, zv));
+        rz.x += d.x;
+        rz.y += d.y;
+    }
+    const int parts = kCgThreads / F, pi = tid / F;
+    red[tid] = rz;
+    __syncthreads();
+    for (int h = parts / 2; h >= 1; h >>= 1) {
+        if (pi < h) {
+            red[tid].x += red[tid + h * F].x;
+            red[tid].y += red[tid + h * F].y;
+        }
+        __syncthreads();
+    }
+    if (tid < F) st.part[int64_t(blockIdx.x) * 2 * F + tid] = red[tid];
+}
+
+// undo the last refinement step of the flagged systems: x64 -= ||r64_prev|| dx
+template <typename R, int F>
+__global__ void k_cg_rollback(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm,
+                              const int* __restrict__ flag, double2* __restrict__ x) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nq || !flag[k % F]) return;
+    const double sc = sqrt(fmax(nrm[k % F].x, 0.0));
+    const double2 d = to_d(dx[k]);
+    x[k].x -= sc * d.x;
+    x[k].y -= sc * d.y;
+}
+
 template <typename R, int F>
 int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* k2_h, const double2* gam_h, int nbc,
              const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
@@ -1253,6 +1358,12 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     DBuf<cplx<R>> gam;
     DBuf<double2> gam64;
     for (DBuf<cplx<R>>* b : {&pv, &q, &r, &z, &dx, &dinv}) EMG_HIP(c, b->alloc(nq));
+    const bool aux = op.aux_n > 0;
+    DBuf<cplx<R>> tg, dginv;
+    if (aux) {
+        EMG_HIP(c, tg.alloc(op.aux_n * F));
+        EMG_HIP(c, dginv.alloc(op.aux_n * F));
+    }
     EMG_HIP(c, r64.alloc(nq));
     EMG_HIP(c, t64.alloc(nq));
     for (DBuf<double2>* b : {&nrm, &bn, &rho, &beta, &alpha}) EMG_HIP(c, b->alloc(F));
@@ -1294,7 +1405,15 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     const unsigned ga = unsigned(8 * std::min<int64_t>(kCgMaxBlocks / 8, std::max<int64_t>(1, (n + 127) / 128)));
     const unsigned gb = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
     k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
+    const unsigned gaux = aux ? grid_for(op.aux_n * F, 256) : 1;
+    if (aux) k_aux_dg<R, F><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv.p);
     EMG_HIP(c, hipGetLastError());
+    // z = M r with M = D^-1 (+ G D_g^-1 G^T)
+    auto precondition = [&]() {
+        if (!aux) return;
+        k_aux_gt<R, F><<<gaux, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
+        k_aux_z<R, F><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg.p, z.p, cs);
+    };
     auto dots64 = [&](const double2* a, double2* out) {
         k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, a, partial.p);
         k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
@@ -1334,7 +1453,17 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     hipGraphExec_t& graph = res.g;
     const int blk = std::max(2, (P.check_every + 1) & ~1);
     std::vector<double> ht(F);
+    // refinement guard: a system whose FP64 residual grows past 4x its best,
+    // or has not improved for 3 cycles, is given up (reported unconverged)
+    // (a step that made the residual worse is undone first)
+    std::vector<double> best_out(F, HUGE_VAL), prev_rel(F, 1.0);
+    std::vector<int> cyc_best(F, 0), gave_up(F, 0), rb(F, 0);
+    DBuf<double2> nrm_prev;
+    DBuf<int> rbd;
+    EMG_HIP(c, nrm_prev.alloc(F));
+    EMG_HIP(c, rbd.alloc(F));
     for (int cyc = 0; cyc < P.max_cycles; ++cyc) {
+        if (cyc > 0) EMG_HIP(c, hipMemcpyAsync(nrm_prev.p, nrm.p, sizeof(double2) * F, hipMemcpyDeviceToDevice, s));
         k_spmv<double, F><<<grow, 256, 0, s>>>(op64, k264.p, gam64.p, x64, nullptr, t64.p);
         k_sub<<<gq, 256, 0, s>>>(nq, b64, t64.p, r64.p);
         dots64(r64.p, nrm.p);
@@ -1345,7 +1474,21 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         for (int f = 0; f < F; ++f) {
             const double rn = std::sqrt(std::max(hr[f].x, 0.0));
             st.relres[f] = bnorm[f] > 0 ? rn / bnorm[f] : 0.0;
-            const bool ok = st.relres[f] <= P.tol;
+            rb[f] = 0;
+            if (!gave_up[f]) {
+                if (st.relres[f] < best_out[f]) {
+                    best_out[f] = st.relres[f];
+                    cyc_best[f] = cyc;
+                } else if (!(st.relres[f] <= 4.0 * best_out[f])) {
+                    gave_up[f] = 1;
+                    rb[f] = 1;
+                    st.relres[f] = prev_rel[f];
+                } else if (cyc - cyc_best[f] >= 3) {
+                    gave_up[f] = 1;
+                }
+            }
+            prev_rel[f] = st.relres[f];
+            const bool ok = st.relres[f] <= P.tol || gave_up[f];
             if (!ok) all = false;
             // inner target relative to the scaled residual (norm 1); a
             // finished system gets a target it already meets
@@ -1353,17 +1496,25 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             ht[f] = t * t;
         }
         st.cycles = cyc;
+        if (std::find(rb.begin(), rb.end(), 1) != rb.end()) {
+            EMG_HIP(c, hipMemcpyAsync(rbd.p, rb.data(), sizeof(int) * F, hipMemcpyHostToDevice, s));
+            k_cg_rollback<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, rbd.p, x64);
+            EMG_HIP(c, hipGetLastError());
+        }
         if (all) break;
         EMG_HIP(c, hipMemcpyAsync(tgt2.p, ht.data(), sizeof(double) * F, hipMemcpyHostToDevice, s));
         k_cast_scaled<R, F><<<gq, 256, 0, s>>>(nq, r64.p, nrm.p, r.p);
-        k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 1);
+        k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 1, int(aux));
+        precondition();
         k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 1);
         EMG_HIP(c, hipGetLastError());
         auto iterations = [&]() {
             for (int i = 0; i < blk; ++i) {
-                k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, z.p, pv.p, q.p, cs);
+                k_cg_pupd<R, F><<<gq, 256, 0, s>>>(nq, z.p, pv.p, cs);
+                k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, pv.p, q.p, cs);
                 k_cg_alpha<F><<<1, kCgThreads, 0, s>>>(int(ga), cs);
-                k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0);
+                k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0, int(aux));
+                precondition();
                 k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 0);
             }
         };
@@ -1457,6 +1608,76 @@ int pow2_block(int64_t remaining, int fblock) {
 
 }  // namespace
 
+namespace {
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = float(a[k]);
+}
+}  // namespace
+
+extern "C" int emg_sweep_escalated(emg_ctx* c, int64_t* n) {
+    EMG_CHECK_CTX(c);
+    if (!n) return fail(c, EMG_ERR_ARG, "emg_sweep_escalated: null output");
+    *n = c->n_escalated;
+    return EMG_OK;
+}
+
+extern "C" int emg_aux_set(emg_ctx* c, int64_t n_nodal, const int32_t* g_ptr, const int32_t* g_idx,
+                           const double* g_val, const int32_t* gt_ptr, const int32_t* gt_idx, const double* gt_val,
+                           const double* dvol, int n_surf, const double* dsurf) {
+    EMG_CHECK_CTX(c);
+    if (n_nodal == 0) {
+        c->aux_n = 0;
+        c->aux_nsurf = 0;
+        return EMG_OK;
+    }
+    if (!c->have_system) return fail(c, EMG_ERR_STATE, "emg_aux_set: no reduced system");
+    if (n_nodal < 0 || n_nodal >= (int64_t(1) << 31) || !g_ptr || !g_idx || !g_val || !gt_ptr || !gt_idx || !gt_val ||
+        !dvol || n_surf < 0 || (n_surf > 0 && !dsurf))
+        return fail(c, EMG_ERR_ARG, "emg_aux_set: bad arguments");
+    const int64_t n = c->n_red;
+    const int64_t nnz = g_ptr[n];
+    if (g_ptr[0] != 0 || nnz < 0 || gt_ptr[0] != 0 || gt_ptr[n_nodal] != nnz)
+        return fail(c, EMG_ERR_ARG, "emg_aux_set: G and G^T row pointers disagree");
+    for (int64_t i = 0; i < n; ++i)
+        if (g_ptr[i + 1] < g_ptr[i]) return fail(c, EMG_ERR_ARG, "emg_aux_set: G row pointers not monotone");
+    for (int64_t j = 0; j < n_nodal; ++j)
+        if (gt_ptr[j + 1] < gt_ptr[j]) return fail(c, EMG_ERR_ARG, "emg_aux_set: G^T row pointers not monotone");
+    for (int64_t e = 0; e < nnz; ++e)
+        if (g_idx[e] < 0 || g_idx[e] >= n_nodal || gt_idx[e] < 0 || gt_idx[e] >= n)
+            return fail(c, EMG_ERR_ARG, "emg_aux_set: column index out of range");
+    hipStream_t s = c->stream;
+    const int64_t nz = std::max<int64_t>(nnz, 1);
+    EMG_HIP(c, c->aux_gptr.alloc(n + 1));
+    EMG_HIP(c, c->aux_tptr.alloc(n_nodal + 1));
+    EMG_HIP(c, c->aux_gidx.alloc(nz));
+    EMG_HIP(c, c->aux_tidx.alloc(nz));
+    EMG_HIP(c, c->aux_gval64.alloc(nz));
+    EMG_HIP(c, c->aux_tval64.alloc(nz));
+    EMG_HIP(c, c->aux_gval32.alloc(nz));
+    EMG_HIP(c, c->aux_tval32.alloc(nz));
+    EMG_HIP(c, c->aux_dvol.alloc(2 * n_nodal));
+    EMG_HIP(c, c->aux_dsurf.alloc(std::max<int64_t>(1, int64_t(n_surf) * n_nodal)));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_gptr.p, g_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_tptr.p, gt_ptr, sizeof(int32_t) * (n_nodal + 1), hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_gidx.p, g_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_tidx.p, gt_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_gval64.p, g_val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_tval64.p, gt_val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_dvol.p, dvol, sizeof(double) * 2 * n_nodal, hipMemcpyHostToDevice, s));
+    if (n_surf > 0)
+        EMG_HIP(c, hipMemcpyAsync(c->aux_dsurf.p, dsurf, sizeof(double) * n_surf * n_nodal, hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        k_to_f32<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, c->aux_gval64.p, c->aux_gval32.p);
+        k_to_f32<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, c->aux_tval64.p, c->aux_tval32.p);
+    }
+    EMG_HIP(c, hipGetLastError());
+    EMG_HIP(c, hipStreamSynchronize(s));
+    c->aux_n = n_nodal;
+    c->aux_nsurf = n_surf;
+    return EMG_OK;
+}
+
 extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, const double* gamma,
                                const double* rhs_coef, int precision, int fblock, double tol, int restart,
                                int max_cycles, double* x_out, double* relres_out, int64_t* iters_out,
@@ -1475,6 +1696,7 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     hipStream_t s = c->stream;
     const int64_t n = c->n_red;
     const int nbc = c->n_bc, np = c->n_forcing;
+    c->n_escalated = 0;
     Op<double> op64;
     op64.n = n;
     op64.ptr = c->red_ptr.p;
@@ -1513,6 +1735,25 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     }
     Op<double> op64s = op64;
     if (c->krylov_method == 2) op64s.B = reinterpret_cast<const double2*>(c->Bs64.p);
+    if (c->krylov_method == 2 && c->aux_n > 0) {
+        if (c->aux_nsurf != nbc)
+            return fail(c, EMG_ERR_STATE, "emg_sweep_solve: auxiliary space has %d surface diagonals, system has %d",
+                        c->aux_nsurf, nbc);
+        auto wire = [&](auto& o, auto gv, auto tv) {
+            o.aux_n = c->aux_n;
+            o.aux_nsurf = c->aux_nsurf;
+            o.gptr = c->aux_gptr.p;
+            o.gidx = c->aux_gidx.p;
+            o.gval = gv;
+            o.tptr = c->aux_tptr.p;
+            o.tidx = c->aux_tidx.p;
+            o.tval = tv;
+            o.dvol = reinterpret_cast<const double2*>(c->aux_dvol.p);
+            o.dsurf = c->aux_dsurf.p;
+        };
+        wire(op32, c->aux_gval32.p, c->aux_tval32.p);
+        wire(op64s, c->aux_gval64.p, c->aux_tval64.p);
+    }
     SolveParams P;
     P.tol = tol;
     P.restart = restart;
@@ -1541,6 +1782,24 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
                            ? dispatch_batch<float>(c, F, op32, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st)
                            : dispatch_batch<double>(c, F, op64s, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st);
         if (rc != EMG_OK) return rc;
+        if (precision == 1 && c->krylov_method == 2) {
+            // systems the complex64 inner solves could not finish continue with
+            // complex128 inner solves from their current iterate (the
+            // refinement recomputes b - A x in FP64; finished systems skip)
+            int bad = 0;
+            for (int f = 0; f < F; ++f) bad += !(st.relres[f] <= tol);
+            if (bad > 0) {
+                BatchStats st2;
+                const int rc2 = dispatch_batch<double>(c, F, op64s, op64, k2 + done, gam.data(), nbc, b64.p, x64.p,
+                                                       P, st2);
+                if (rc2 != EMG_OK) return rc2;
+                for (int f = 0; f < F; ++f) {
+                    st.iters[f] += st2.iters[f];
+                    st.relres[f] = st2.relres[f];
+                }
+                c->n_escalated += bad;
+            }
+        }
         if (overlap_out) {
             std::vector<double2> ov;
             const int orc = port_overlaps(c, F, x64.p, ov);

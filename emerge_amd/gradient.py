@@ -1,0 +1,138 @@
+"""Discrete gradient from second-order nodal (Lagrange) functions into the
+second-order Nedelec space, and the diagonal of the operator on that
+gradient space.  Host-side set-up of the auxiliary-space preconditioner of
+the Krylov solve (the solve applies ``D^-1 + G D_g^-1 G^T``).
+
+Basis (E/elements/ned2_interp.py:243-296, in the global-id sorted frame of
+each edge x<y and face p<q<r; W_ab = lam_a grad lam_b - lam_b grad lam_a):
+
+* edge DOFs (1st, 2nd kind): ``-|xy| lam_x W_xy``, ``-|xy| lam_y W_xy``;
+* face DOFs (1st, 2nd kind): ``|pr| lam_q W_pr``, ``-|pq| lam_r W_pq``;
+  the third face function follows from
+  ``lam_p W_qr = lam_q W_pr - lam_r W_pq``.
+
+Nodal space: vertex hats ``lam_a`` (column a) and edge bubbles
+``lam_x lam_y`` (column V + e).  With ``sum_v lam_v = 1`` on every tet,
+
+* ``grad lam_a = sum_{edges ab} W_ba`` and
+  ``W_xy = lam_x W_xy + lam_y W_xy + sum_{faces xyc} lam_c W_xy``;
+* ``grad(lam_x lam_y) = (lam_x - lam_y) W_xy
+  + sum_{faces xyc} (lam_x W_cy + lam_y W_cx)``,
+
+every term of which is one edge or face DOF function (or the face
+identity above), so ``G`` is exact: the curl-curl matrix annihilates it.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+
+def discrete_gradient(mesh, basis) -> sp.csr_matrix:
+    """(N, V + nE) real CSR: column a is grad lam_a, column V + e is
+    grad(lam_x lam_y) of edge e = (x, y), in the global DOF numbering."""
+    V, nE, nF = mesh.n_nodes, mesh.n_edges, mesh.n_tris
+    N = basis.n_field
+    L = mesh.edge_lengths
+    e1 = basis.edge_to_field[0]
+    e2 = basis.edge_to_field[1]
+    f1 = np.arange(nF) + nE
+    f2 = np.arange(nF) + 2 * nE + nF
+    t2e = mesh.tri_to_edge              # (pq, qr, pr) of the sorted triangle
+    s1 = L[t2e[2]]                      # |pr|
+    s2 = L[t2e[0]]                      # |pq|
+    tri = np.arange(nF)
+
+    # Wexp (N, nE): W_xy of every edge in the Nedelec-2 basis
+    rows = [e1, e2]
+    cols = [np.arange(nE), np.arange(nE)]
+    vals = [-1.0 / L, -1.0 / L]
+    # lam_c W_xy for the faces through the edge; edge k of the face:
+    # k=0 (p,q), c=r: -F2/s2;  k=1 (q,r), c=p: F1/s1 + F2/s2;  k=2 (p,r), c=q: F1/s1
+    wf = {0: (0.0 * s1, -1.0 / s2), 1: (1.0 / s1, 1.0 / s2), 2: (1.0 / s1, 0.0 * s2)}
+    # grad(lam_x lam_y) face part: k=0: (-2/s1, -1/s2); k=1: (1/s1, -1/s2); k=2: (1/s1, 2/s2)
+    bf = {0: (-2.0 / s1, -1.0 / s2), 1: (1.0 / s1, -1.0 / s2), 2: (1.0 / s1, 2.0 / s2)}
+    brows = [e1, e2]
+    bcols = [np.arange(nE), np.arange(nE)]
+    bvals = [-1.0 / L, 1.0 / L]
+    for k in range(3):
+        e = t2e[k]
+        a1, a2 = wf[k]
+        rows += [f1, f2]
+        cols += [e, e]
+        vals += [a1, a2]
+        b1, b2 = bf[k]
+        brows += [f1, f2]
+        bcols += [e, e]
+        bvals += [b1, b2]
+    Wexp = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, nE))
+    # grad lam_a = sum over edges (x, y): +W_xy where a = y, -W_xy where a = x
+    inc = sp.csr_matrix((np.concatenate([np.ones(nE), -np.ones(nE)]),
+                         (np.concatenate([np.arange(nE), np.arange(nE)]),
+                          np.concatenate([mesh.edges[1], mesh.edges[0]]))), shape=(nE, V))
+    Gv = (Wexp @ inc).tocsr()
+    Ge = sp.csr_matrix((np.concatenate(bvals), (np.concatenate(brows), np.concatenate(bcols))), shape=(N, nE))
+    G = sp.hstack([Gv, Ge], format="csr")
+    G.sum_duplicates()
+    G.eliminate_zeros()
+    return G
+
+
+def _tet_gradients(nodes: np.ndarray, tets: np.ndarray):
+    """Barycentric gradients (T, 4, 3) and volumes (T,) of every tet."""
+    X = nodes[:, tets]                                   # (3, 4, T)
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=1)  # (3, 3, T)
+    J = np.moveaxis(J, 2, 0)                             # (T, 3 coord, 3 edge)
+    det = np.linalg.det(J)
+    inv = np.linalg.inv(J)                               # (T, 3 edge, 3 coord): rows grad lam_1..3
+    g = np.empty((tets.shape[1], 4, 3))
+    g[:, 1:] = inv
+    g[:, 0] = -inv.sum(axis=1)
+    return g, np.abs(det) / 6.0
+
+
+def nodal_volume_diagonal(mesh, eps_r: np.ndarray) -> np.ndarray:
+    """int eps |grad phi_j|^2 dV for every nodal function (V + nE,), with
+    eps the tet's (scalar) permittivity: vertex hats sum eps V |grad lam|^2,
+    edge bubbles eps V/10 (|g_x|^2 + |g_y|^2 + g_x . g_y)."""
+    V, nE = mesh.n_nodes, mesh.n_edges
+    g, vol = _tet_gradients(mesh.nodes, mesh.tets)
+    w = eps_r * vol                                      # (T,) complex
+    out = np.zeros(V + nE, dtype=np.complex128)
+    gg = np.einsum("tic,tjc->tij", g, g)                 # (T, 4, 4)
+    for i in range(4):
+        np.add.at(out, mesh.tets[i], w * gg[:, i, i])
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    for i, j in pairs:
+        eid = mesh.edge_ids(mesh.tets[i], mesh.tets[j])
+        val = w / 10.0 * (gg[:, i, i] + gg[:, j, j] + gg[:, i, j])
+        np.add.at(out, V + eid, val)
+    return out
+
+
+def nodal_surface_diagonal(mesh, tri_ids: np.ndarray) -> np.ndarray:
+    """int_S |grad_t phi_j|^2 dS over the given triangles (V + nE,): vertex
+    hats A |g|^2, edge bubbles A/6 (|g_x|^2 + |g_y|^2 + g_x . g_y), with g the
+    in-plane barycentric gradients of each triangle."""
+    V, nE = mesh.n_nodes, mesh.n_edges
+    out = np.zeros(V + nE)
+    if len(tri_ids) == 0:
+        return out
+    t = mesh.tris[:, tri_ids]                            # (3, n)
+    X = mesh.nodes[:, t]                                 # (3 coord, 3 vert, n)
+    a = X[:, 1] - X[:, 0]
+    b = X[:, 2] - X[:, 0]
+    n = np.cross(a, b, axis=0)
+    area2 = np.sqrt((n ** 2).sum(axis=0))
+    A = area2 / 2
+    # in-plane gradients: grad lam_i = (n x e_i) / |n|^2 with e_i the edge opposite i
+    nn = n / area2 ** 2
+    opp = [X[:, 2] - X[:, 1], X[:, 0] - X[:, 2], X[:, 1] - X[:, 0]]
+    g = [np.cross(nn, e, axis=0) for e in opp]
+    for i in range(3):
+        np.add.at(out, t[i], A * (g[i] ** 2).sum(axis=0))
+    for i, j in [(0, 1), (1, 2), (0, 2)]:
+        eid = mesh.edge_ids(t[i], t[j])
+        val = A / 6.0 * ((g[i] ** 2).sum(axis=0) + (g[j] ** 2).sum(axis=0) + (g[i] * g[j]).sum(axis=0))
+        np.add.at(out, V + eid, val)
+    return out

@@ -142,6 +142,14 @@ class Mesh3D:
             raise ValueError(f"There is no edge with indices {i1}, {i2}")
         return idx
 
+    def edge_ids(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """Vectorised ``get_edge``: edge ids of the vertex pairs (a, b)."""
+        key = _pair_key(np.asarray(a, dtype=np.int64), np.asarray(b, dtype=np.int64), self.n_nodes)
+        idx = np.searchsorted(self._edge_keys, key)
+        if np.any(idx >= self._edge_keys.shape[0]) or np.any(self._edge_keys[np.minimum(idx, self._edge_keys.shape[0] - 1)] != key):
+            raise ValueError("edge_ids: a vertex pair is not an edge of the mesh")
+        return idx
+
     def get_tri(self, i1: int, i2: int, i3: int) -> int:
         nv = self.n_nodes
         a, b, c = sorted((int(i1), int(i2), int(i3)))
@@ -162,6 +170,16 @@ class Mesh3D:
         on = np.all(np.abs(coords - value) <= tol, axis=0)
         return bt[on]
 
+    def tris_in_box(self, lo, hi, tol: float = 1e-9) -> np.ndarray:
+        """All triangles (interior ones included) whose three vertices lie in
+        the closed axis-aligned box [lo, hi]; a degenerate box (lo == hi on
+        one axis) selects a planar patch, e.g. an internal PEC sheet."""
+        lo = np.asarray(lo, dtype=np.float64)[:, None, None]
+        hi = np.asarray(hi, dtype=np.float64)[:, None, None]
+        c = self.nodes[:, self.tris]  # (3, 3, nF)
+        inside = np.all((c >= lo - tol) & (c <= hi + tol), axis=(0, 1))
+        return np.nonzero(inside)[0]
+
     def tri_nodes(self, tri_ids: np.ndarray) -> np.ndarray:
         return np.unique(self.tris[:, tri_ids].reshape(-1))
 
@@ -173,30 +191,25 @@ class Mesh3D:
         return np.nonzero(mask[self.tets].any(axis=0))[0]
 
 
-def kuhn_box(nx: int, ny: int, nz: int,
-             size: tuple[float, float, float] = (1.0, 1.0, 1.0),
-             origin: tuple[float, float, float] = (0.0, 0.0, 0.0),
-             jitter: float = 0.0, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
-    """Box of nx*ny*nz hexahedra, each split into 6 Kuhn tets.
+def kuhn_grid(xs: np.ndarray, ys: np.ndarray, zs: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Tensor-product grid with coordinate lines ``xs``, ``ys``, ``zs``
+    (strictly increasing), each cell split into 6 Kuhn tets.
 
-    Vertex ``(i, j, k)`` has id ``i + (nx+1)*(j + (ny+1)*k)``.  Interior vertices
-    move by ``U[-jitter, jitter] * h`` per axis (PCG64 stream ``seed``); a Kuhn
-    tet's smallest altitude is h/sqrt(2), so ``jitter < 0.2`` cannot invert one.
-    Every tet is positively oriented.  Returns ``(nodes (3,V), tets (4,T))``.
+    Vertex ``(i, j, k)`` has id ``i + nx1*(j + ny1*k)`` (``nx1 = len(xs)``);
+    the 6 tets of a cell are consecutive, cells ordered x fastest.  Every tet
+    is positively oriented.  Returns ``(nodes (3,V), tets (4,T))``.
     """
-    hx, hy, hz = size[0] / nx, size[1] / ny, size[2] / nz
+    xs, ys, zs = (np.asarray(a, dtype=np.float64) for a in (xs, ys, zs))
+    for a in (xs, ys, zs):
+        if a.ndim != 1 or a.size < 2 or np.any(np.diff(a) <= 0):
+            raise ValueError("kuhn_grid: coordinate lines must be strictly increasing")
+    nx, ny, nz = xs.size - 1, ys.size - 1, zs.size - 1
     ii, jj, kk = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
                              indexing="ij")
     ii = ii.transpose(2, 1, 0).reshape(-1)
     jj = jj.transpose(2, 1, 0).reshape(-1)
     kk = kk.transpose(2, 1, 0).reshape(-1)
-    nodes = np.stack([origin[0] + ii * hx, origin[1] + jj * hy, origin[2] + kk * hz])
-    if jitter > 0.0:
-        rng = np.random.default_rng(seed)
-        shift = rng.uniform(-jitter, jitter, size=nodes.shape)
-        shift *= np.array([hx, hy, hz])[:, None]
-        interior = (ii > 0) & (ii < nx) & (jj > 0) & (jj < ny) & (kk > 0) & (kk < nz)
-        nodes[:, interior] += shift[:, interior]
+    nodes = np.stack([xs[ii], ys[jj], zs[kk]])
 
     def vid(i, j, k):
         return i + (nx + 1) * (j + (ny + 1) * k)
@@ -218,6 +231,32 @@ def kuhn_box(nx: int, ny: int, nz: int,
         if sgn < 0:
             v[1], v[2] = v[2], v[1]
         tets.append(np.stack(v))
-    # hex-major, Kuhn-permutation minor: the 6 tets of one hex are consecutive
+    # cell-major, Kuhn-permutation minor: the 6 tets of one cell are consecutive
     tets = np.stack(tets, axis=2).reshape(4, -1)
+    return nodes, tets
+
+
+def kuhn_box(nx: int, ny: int, nz: int,
+             size: tuple[float, float, float] = (1.0, 1.0, 1.0),
+             origin: tuple[float, float, float] = (0.0, 0.0, 0.0),
+             jitter: float = 0.0, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """Box of nx*ny*nz hexahedra, each split into 6 Kuhn tets.
+
+    Vertex ``(i, j, k)`` has id ``i + (nx+1)*(j + (ny+1)*k)``.  Interior vertices
+    move by ``U[-jitter, jitter] * h`` per axis (PCG64 stream ``seed``); a Kuhn
+    tet's smallest altitude is h/sqrt(2), so ``jitter < 0.2`` cannot invert one.
+    Every tet is positively oriented.  Returns ``(nodes (3,V), tets (4,T))``.
+    """
+    hx, hy, hz = size[0] / nx, size[1] / ny, size[2] / nz
+    nodes, tets = kuhn_grid(origin[0] + np.arange(nx + 1) * hx, origin[1] + np.arange(ny + 1) * hy,
+                            origin[2] + np.arange(nz + 1) * hz)
+    if jitter > 0.0:
+        ii = np.tile(np.arange(nx + 1), (ny + 1) * (nz + 1))
+        jj = np.tile(np.repeat(np.arange(ny + 1), nx + 1), nz + 1)
+        kk = np.repeat(np.arange(nz + 1), (nx + 1) * (ny + 1))
+        rng = np.random.default_rng(seed)
+        shift = rng.uniform(-jitter, jitter, size=nodes.shape)
+        shift *= np.array([hx, hy, hz])[:, None]
+        interior = (ii > 0) & (ii < nx) & (jj > 0) & (jj < ny) & (kk > 0) & (kk < nz)
+        nodes[:, interior] += shift[:, interior]
     return nodes, tets

@@ -21,6 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import bc as bcs
+from . import gradient
 from ._lib import Context, ptr
 from .assembler import MeshOnDevice
 
@@ -53,6 +54,8 @@ class SweepResult:
     precision: str
     S: np.ndarray | None = None   # (nfreq, nports, nports), rows = receiving port
     timing: dict = field(default_factory=dict)
+    fp64_fallback: np.ndarray | None = None  # (nfreq, nports) solved again in FP64 from zero
+    fp64_escalated: int = 0   # mixed systems finished with complex128 inner solves (warm)
 
 
 def apply_s_caps(Sp: np.ndarray, cap_single: bool = True, cap_col: bool = True, recip: bool = False):
@@ -73,6 +76,18 @@ def apply_s_caps(Sp: np.ndarray, cap_single: bool = True, cap_col: bool = True, 
     return Sp
 
 
+def _morton_order(pos: np.ndarray, ids: np.ndarray, h: float) -> np.ndarray:
+    """Permutation sorting points (3, n) along a Morton curve of cells of
+    size h (ties by id)."""
+    lo = pos.min(axis=1, keepdims=True)
+    q = np.clip(np.floor((pos - lo) / max(h, 1e-300)), 0, 1023).astype(np.int64)
+    key = np.zeros(pos.shape[1], dtype=np.int64)
+    for b in range(10):
+        for d in range(3):
+            key |= ((q[d] >> b) & 1) << (3 * b + d)
+    return np.lexsort((ids, key))
+
+
 class FrequencySweep:
     """Device-resident E, B, surface terms and forcing of one mesh."""
 
@@ -86,6 +101,9 @@ class FrequencySweep:
         self.ctx = ctx if ctx is not None else Context(device)
         self.dm = MeshOnDevice(self.ctx, mesh.nodes, mesh.tets, basis.tet_to_field, basis.n_field)
         self.dm.assemble(eps_r, inv_mu_r)
+        self.eps_r = np.asarray(eps_r, dtype=np.complex128)
+        self._aux = None          # host arrays of the auxiliary gradient space, built on first use
+        self._aux_on = False
         self.bcs = list(boundary_conditions)
         self.pec = [b for b in self.bcs if isinstance(b, bcs.PEC)]
         self.robin = [b for b in self.bcs if isinstance(b, bcs.RobinBC)]
@@ -109,18 +127,11 @@ class FrequencySweep:
         midpoints, face centroids; both DOFs of an entity stay adjacent), so
         that consecutive reduced rows -- and each XCD's contiguous share of
         them -- couple to nearby columns."""
-        m, f = self.mesh, self.basis
+        m = self.mesh
         ec = 0.5 * (m.nodes[:, m.edges[0]] + m.nodes[:, m.edges[1]])
         fc = (m.nodes[:, m.tris[0]] + m.nodes[:, m.tris[1]] + m.nodes[:, m.tris[2]]) / 3.0
         pos = np.concatenate([ec, fc, ec, fc], axis=1)[:, ids]
-        lo = pos.min(axis=1, keepdims=True)
-        h = max(float(np.median(m.edge_lengths)), 1e-300)
-        q = np.clip(np.floor((pos - lo) / h), 0, 1023).astype(np.int64)
-        key = np.zeros(ids.shape[0], dtype=np.int64)
-        for b in range(10):
-            for d in range(3):
-                key |= ((q[d] >> b) & 1) << (3 * b + d)
-        return ids[np.lexsort((ids, key))]
+        return ids[_morton_order(pos, ids, float(np.median(m.edge_lengths)))]
 
     def _setup(self) -> None:
         f, m = self.basis, self.mesh
@@ -257,6 +268,57 @@ class FrequencySweep:
             self.ctx.call("emg_port_norms_get", ptr(norms))
             self.port_norms = norms
 
+    def aux_space(self):
+        """Auxiliary gradient space of the COCG preconditioner
+        ``D^-1 + G D_g^-1 G^T``: G = the discrete gradient of the second-order
+        nodal functions (``emerge_amd.gradient``) restricted to the reduced
+        rows and to the nodal functions that vanish on PEC (their gradients
+        have no tangential part there), columns in Morton order; D_g from the
+        exact volume integral int eps |grad phi|^2 and, per Robin term,
+        int_S |grad_t phi|^2 (second-order ABC corrections contribute none).
+        Returns (G csr, dvol, dsurf)."""
+        if self._aux is None:
+            m, f = self.mesh, self.basis
+            G = gradient.discrete_gradient(m, f)
+            V = m.n_nodes
+            drop = np.zeros(G.shape[1], dtype=bool)
+            for p in self.pec:
+                drop[np.unique(m.tris[:, p.tri_ids])] = True
+                drop[V + np.unique(m.tri_to_edge[:, p.tri_ids])] = True
+            cols = np.nonzero(~drop)[0]
+            mid = 0.5 * (m.nodes[:, m.edges[0]] + m.nodes[:, m.edges[1]])
+            pos = np.concatenate([m.nodes, mid], axis=1)[:, cols]
+            cols = cols[_morton_order(pos, cols, float(np.median(m.edge_lengths)))]
+            Gr = G[self.solve_ids][:, cols].tocsr()
+            Gr.sort_indices()
+            dvol = gradient.nodal_volume_diagonal(m, self.eps_r)[cols]
+            dsurf = np.zeros((len(self.surface), cols.shape[0]))
+            for i, (kind, b) in enumerate(self.surface):
+                if kind == "robin":
+                    dsurf[i] = gradient.nodal_surface_diagonal(m, b.tri_ids)[cols]
+            self._aux = (Gr, dvol, dsurf)
+        return self._aux
+
+    def _set_aux(self, on: bool) -> None:
+        if on == self._aux_on:
+            return
+        if not on:
+            self.ctx.call("emg_aux_set", 0, None, None, None, None, None, None, None, 0, None)
+            self._aux_on = False
+            return
+        Gr, dvol, dsurf = self.aux_space()
+        Gt = Gr.T.tocsr()
+        Gt.sort_indices()
+        gp, gi = np.ascontiguousarray(Gr.indptr, np.int32), np.ascontiguousarray(Gr.indices, np.int32)
+        gv = np.ascontiguousarray(Gr.data, np.float64)
+        tp, ti = np.ascontiguousarray(Gt.indptr, np.int32), np.ascontiguousarray(Gt.indices, np.int32)
+        tv = np.ascontiguousarray(Gt.data, np.float64)
+        dv = np.ascontiguousarray(dvol, np.complex128)
+        ds = np.ascontiguousarray(dsurf, np.float64)
+        self.ctx.call("emg_aux_set", Gr.shape[1], ptr(gp), ptr(gi), ptr(gv), ptr(tp), ptr(ti), ptr(tv), ptr(dv),
+                      ds.shape[0], ptr(ds) if ds.size else None)
+        self._aux_on = True
+
     def voltage_line_probe(self, port, npts: int = 21):
         """Probe points of a voltage-line port.  Lines (MW/microwave_3d.py:
         336-372): from every port-face node with the smallest coordinate along
@@ -362,27 +424,41 @@ class FrequencySweep:
 
     def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock: int = 16, restart: int = 100,
               max_cycles: int = 200, keep_fields: bool = True, method: str = "gmres", idr_s: int = 4,
-              inner_rel32: float = 1e-5) -> SweepResult:
+              inner_rel32: float = 1e-5, precond: str = "aux", fallback: bool = True) -> SweepResult:
         """Solve every (frequency, active port) system.  method 'gmres' =
         GMRES(restart); 'idr' = IDR(idr_s) with a budget of `restart`
         matvecs per FP64 refinement cycle; 'cocg' = COCG on the symmetric
         part inside FP64 refinement, at most `restart` iterations per cycle,
-        each cycle reducing the residual by `inner_rel32`."""
+        each cycle reducing the residual by `inner_rel32`.  precond (COCG
+        only): 'aux' = Jacobi plus the auxiliary gradient-space correction
+        (``aux_space``), 'jacobi' = Jacobi alone.  fallback (mixed only):
+        systems that miss tol are solved again in FP64
+        (``SweepResult.fp64_fallback`` marks them)."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+        if precond not in ("aux", "jacobi"):
+            raise ValueError("precond must be 'aux' or 'jacobi'")
+        self._set_aux(method == "cocg" and precond == "aux")
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s),
                       float(inner_rel32))
         k2, gam, coef = self.batch_coefficients(freqs)
-        nbatch = k2.shape[0]
-        x = np.empty((nbatch, self.n_red), dtype=np.complex128) if keep_fields else None
-        relres = np.empty(nbatch)
-        iters = np.empty(nbatch, dtype=np.int64)
         npt = len(self.ports)
-        ov = np.empty((nbatch, npt), dtype=np.complex128) if self.port_norms is not None else None
-        prec = {"fp64": 0, "mixed": 1}[precision]
-        self.ctx.call("emg_sweep_solve", nbatch, ptr(k2), ptr(gam) if gam.size else None, ptr(coef), prec,
-                      int(fblock), float(tol), int(restart), int(max_cycles),
-                      ptr(x) if x is not None else None, ptr(relres), ptr(iters),
-                      ptr(ov) if ov is not None else None)
+        self._escalated = 0
+        run = dict(fblock=fblock, tol=tol, restart=restart, max_cycles=max_cycles, keep_fields=keep_fields)
+        x, relres, iters, ov = self._run_rows(k2, gam, coef, {"fp64": 0, "mixed": 1}[precision], **run)
+        fell_back = np.zeros(k2.shape[0], dtype=bool)
+        if precision == "mixed" and fallback:
+            # systems the complex64 inner solves could not bring to tol (the
+            # refinement gave them up) are solved again in FP64
+            bad = np.nonzero(~(relres <= tol))[0]
+            if bad.size:
+                xb, rb, ib, ob = self._run_rows(k2[bad], gam[bad] if gam.size else gam, coef[bad], 0, **run)
+                relres[bad] = rb
+                iters[bad] += ib
+                if x is not None:
+                    x[bad] = xb
+                if ov is not None:
+                    ov[bad] = ob
+                fell_back[bad] = True
         fields = {}
         if x is not None:
             for i in range(freqs.shape[0]):
@@ -392,4 +468,23 @@ class FrequencySweep:
                     fields[(i, p.port_number)] = full
         S = self.s_matrix(ov, freqs) if ov is not None else None
         return SweepResult(freqs, self.ports, fields, relres.reshape(-1, npt), iters.reshape(-1, npt), precision,
-                           S=S)
+                           S=S, fp64_fallback=fell_back.reshape(-1, npt), fp64_escalated=self._escalated)
+
+    def _run_rows(self, k2, gam, coef, prec, fblock, tol, restart, max_cycles, keep_fields):
+        """One emg_sweep_solve over the given (frequency, port) rows."""
+        nbatch = k2.shape[0]
+        k2 = np.ascontiguousarray(k2)
+        gam = np.ascontiguousarray(gam)
+        coef = np.ascontiguousarray(coef)
+        x = np.empty((nbatch, self.n_red), dtype=np.complex128) if keep_fields else None
+        relres = np.empty(nbatch)
+        iters = np.empty(nbatch, dtype=np.int64)
+        ov = np.empty((nbatch, len(self.ports)), dtype=np.complex128) if self.port_norms is not None else None
+        self.ctx.call("emg_sweep_solve", nbatch, ptr(k2), ptr(gam) if gam.size else None, ptr(coef), prec,
+                      int(fblock), float(tol), int(restart), int(max_cycles),
+                      ptr(x) if x is not None else None, ptr(relres), ptr(iters),
+                      ptr(ov) if ov is not None else None)
+        esc = ctypes.c_int64()
+        self.ctx.call("emg_sweep_escalated", ctypes.byref(esc))
+        self._escalated += esc.value
+        return x, relres, iters, ov

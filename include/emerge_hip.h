@@ -212,6 +212,28 @@ int emg_port_norms_get(emg_ctx* ctx, double* norms);
  * device algorithm. */
 int emg_solver_config(emg_ctx* ctx, int method, int idr_s, double inner_rel32);
 
+/* Number of systems of the last emg_sweep_solve (precision 1, COCG) whose
+ * complex64 inner solves stalled and that were finished with complex128
+ * inner solves from their current iterate.  No reference counterpart (the
+ * reference solves in FP64 throughout). */
+int emg_sweep_escalated(emg_ctx* ctx, int64_t* n);
+
+/* Auxiliary gradient space of the COCG preconditioner (method 2 of
+ * emg_sweep_solve): the inner solves apply z = D^-1 r + G D_g^-1 G^T r with
+ * D the diagonal of the operator, G (n_red x n_nodal, real CSR g_*, its
+ * transpose gt_*) the discrete gradient of the second-order nodal functions
+ * in the reduced Nedelec-2 rows, and per system
+ * D_g = -k0^2 dvol + sum_s gamma_s dsurf[s] (dvol complex n_nodal, dsurf
+ * real [n_surf][n_nodal], one row per surface term of emg_surface_set).
+ * A Maxwell (Hiptmair-type) gradient correction: Jacobi alone leaves the
+ * curl-free near-null space of E - k0^2 B unresolved.  No counterpart in
+ * the reference, whose default path is a direct solve (E/solver.py:
+ * 535-773; its iterative path uses ILU, :437-527).  n_nodal = 0 turns it
+ * off; a new reduced system (emg_system_reduce) clears it. */
+int emg_aux_set(emg_ctx* ctx, int64_t n_nodal, const int32_t* g_ptr, const int32_t* g_idx, const double* g_val,
+                const int32_t* gt_ptr, const int32_t* gt_idx, const double* gt_val, const double* dvol,
+                int n_surf, const double* dsurf);
+
 /* Generic complex CSR system for the Solver plug-in (replaces
  * Solver.solve(A, b, ...), E/solver.py:365-367, next to SuperLU / PARDISO /
  * cuDSS, E/solver.py:535-773): vals complex128 interleaved. */

@@ -241,3 +241,73 @@ def test_waveguide_abc_termination_matches_superlu(order):
         ref = o["fields"][1][ids]
         assert np.linalg.norm(res.fields[(i, 1)][ids] - ref) / np.linalg.norm(ref) <= 1e-6
     print(f"abc order {order}: S11={res.S[:, 0, 0]} iters={res.iters.ravel().tolist()}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sweep_cocg_jacobi_only_matches_superlu(sweep, precision):
+    """COCG with Jacobi alone (no auxiliary gradient space)."""
+    print(f"cocg jacobi precision={precision}", flush=True)
+    freqs = np.array([9e9, 11e9])
+    res = sweep.solve(freqs, precision=precision, tol=1e-8, fblock=2, restart=20000, max_cycles=30,
+                      method="cocg", inner_rel32=1e-2, precond="jacobi")
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = wg_case.oracle_solution(float(f))
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6
+    print(f"cocg jacobi iters={res.iters.tolist()}")
+
+
+def test_sweep_cocg_aux_needs_fewer_iterations(sweep):
+    """The gradient-space correction cuts the COCG iteration count on the
+    waveguide (same systems, same tolerance)."""
+    freqs = np.array([10e9])
+    kw = dict(precision="fp64", tol=1e-8, fblock=1, restart=20000, max_cycles=30, method="cocg", inner_rel32=1e-2)
+    ra = sweep.solve(freqs, precond="aux", **kw)
+    rj = sweep.solve(freqs, precond="jacobi", **kw)
+    assert np.all(ra.relres <= 1e-8) and np.all(rj.relres <= 1e-8)
+    assert ra.iters.sum() * 3 < rj.iters.sum(), (ra.iters, rj.iters)
+    assert np.abs(ra.S - rj.S).max() <= 1e-6
+
+
+def test_mixed_escalates_to_fp64_inner_solves(sweep):
+    """With three refinement cycles the complex64 pass ends short of 1e-8;
+    the unfinished systems continue with complex128 inner solves from their
+    iterate and converge (escalation counted)."""
+    freqs = np.array([8.5e9, 11.5e9])
+    res = sweep.solve(freqs, precision="mixed", tol=1e-8, fblock=2, restart=20000, max_cycles=3,
+                      method="cocg", inner_rel32=1e-2, fallback=False)
+    assert res.fp64_escalated >= 1
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = wg_case.oracle_solution(float(f))
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6
+    print(f"escalated={res.fp64_escalated} iters={res.iters.tolist()}")
+
+
+def test_uneven_meshes_solved_concurrently():
+    """Independent jittered waveguides of uneven size, one device context
+    each, solved from host threads at once: every point converges and the
+    S-parameters equal the one-at-a-time solves bit for bit."""
+    import threading
+    from emerge_amd import workloads as w
+    probs = w.waveguide_family(count=3, n_range=(6_000, 16_000), seed=3)
+    sws = [FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, w.waveguide_bcs(p)) for p in probs]
+    freqs = np.array([9e9, 10e9, 11e9])
+    kw = dict(precision="mixed", tol=1e-8, fblock=8, restart=20000, max_cycles=30, method="cocg",
+              inner_rel32=1e-2, keep_fields=False)
+    seq = [sw.solve(freqs, **kw) for sw in sws]
+    out = [None] * len(sws)
+
+    def run(i):
+        out[i] = sws[i].solve(freqs, **kw)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(sws))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for a, b in zip(seq, out):
+        assert np.all(b.relres <= 1e-8)
+        assert np.array_equal(a.S, b.S)
+        # |S21| -> 1 on the lossless guide above cutoff
+        assert np.all(np.abs(np.abs(b.S[:, 1, 0]) - 1) <= 2e-2)

@@ -1,0 +1,125 @@
+"""Pins of the discrete gradient and nodal diagonals of the auxiliary-space
+preconditioner (emerge_amd.gradient), on the CPU:
+
+* curl-free: the oracle's curl-curl matrix annihilates G (a planted wrong
+  coefficient does not);
+* pointwise: sum_i (G c)_i N_i, with N_i the reference's interpolation basis
+  (oracle.quadrature.basis_values), equals grad(sum_j c_j phi_j) evaluated
+  from barycentric coordinates;
+* diagonals: the closed-form int eps |grad phi|^2 and int_S |grad_t phi|^2
+  equal g^T M g with M the exactly integrated mass matrix (quadrature of the
+  basis) and the pinned Robin matrix.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from emerge_amd import gradient as gr
+from emerge_amd.basis import Nedelec2
+from emerge_amd.mesh import Mesh3D, kuhn_box
+from oracle import curlcurl as cc
+from oracle import quadrature as qd
+from oracle import robin
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    nodes, tets = kuhn_box(3, 2, 2, size=(0.02, 0.015, 0.01), jitter=0.15, seed=5)
+    m = Mesh3D(nodes, tets)
+    return m, Nedelec2(m)
+
+
+def test_gradient_is_curl_free(mesh):
+    m, f = mesh
+    G = gr.discrete_gradient(m, f)
+    assert G.shape == (f.n_field, m.n_nodes + m.n_edges)
+    T = m.n_tets
+    E, _ = cc.tet_mass_stiffness_matrices(m, f.tet_to_field, cc.isotropic(np.ones(T) + 0j),
+                                          cc.isotropic(np.ones(T) + 0j))
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(G.shape[1])
+    Gx = G @ x
+    scale = abs(E).max() * np.linalg.norm(Gx)
+    assert np.linalg.norm(E @ Gx) <= 1e-13 * scale
+    # planted mistake: one face coefficient off by 10% leaves a curl
+    Gb = G.copy().tolil()
+    face_row = m.n_edges + 3
+    j = Gb.rows[face_row][0]
+    Gb[face_row, j] = 1.1 * Gb[face_row, j]
+    Gb = Gb.tocsr()
+    assert np.linalg.norm(E @ (Gb @ x)) > 1e-6 * scale
+
+
+def _bary(verts, pts):
+    """Barycentric coordinates (4, P) and their gradients (4, 3) of a tet."""
+    J = np.stack([verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0], verts[:, 3] - verts[:, 0]], axis=1)
+    Ji = np.linalg.inv(J)
+    l123 = Ji @ (pts - verts[:, :1])
+    lam = np.vstack([1 - l123.sum(axis=0), l123])
+    g = np.vstack([-Ji.sum(axis=0), Ji])
+    return lam, g
+
+
+def test_gradient_matches_nodal_function_pointwise(mesh):
+    m, f = mesh
+    G = gr.discrete_gradient(m, f)
+    rng = np.random.default_rng(1)
+    c = rng.standard_normal(G.shape[1])
+    u = G @ c
+    V = m.n_nodes
+    for t in (0, 7, m.n_tets - 1):
+        vid = m.tets[:, t]
+        verts = m.nodes[:, vid]
+        w = rng.dirichlet(np.ones(4), size=5).T  # (4, 5) interior points
+        pts = verts @ w
+        edges_g = m.edges[:, m.tet_to_edge[:, t]]
+        tris_g = m.tris[:, m.tet_to_tri[:, t]]
+        lem = cc.local_mapping(vid[:, None], edges_g[:, :, None])[:, :, 0]
+        ltm = cc.local_mapping(vid[:, None], tris_g[:, :, None])[:, :, 0]
+        N = qd.basis_values(pts, verts[0], verts[1], verts[2], lem, ltm)  # (20, 3, P)
+        field = np.einsum("i,icp->cp", u[f.tet_to_field[:, t]], N)
+        lam, g = _bary(verts, pts)
+        ref = np.zeros_like(field)
+        for a in range(4):
+            ref += c[vid[a]] * g[a][:, None]
+        for a in range(4):
+            for b in range(a + 1, 4):
+                e = m.get_edge(vid[a], vid[b])
+                ref += c[V + e] * (lam[a] * g[b][:, None] + lam[b] * g[a][:, None])
+        assert np.abs(field - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_nodal_volume_diagonal_is_gMg(mesh):
+    m, f = mesh
+    T = m.n_tets
+    rng = np.random.default_rng(2)
+    eps = rng.uniform(1, 4, T) * (1 - 0.01j)
+    rows, cols, vals = [], [], []
+    for t in range(T):
+        vid = m.tets[:, t]
+        _, Mt = qd.element_matrices_quadrature(m.nodes[:, vid], vid, m.edges[:, m.tet_to_edge[:, t]],
+                                               m.tris[:, m.tet_to_tri[:, t]], np.eye(3), eps[t] * np.eye(3))
+        d = f.tet_to_field[:, t]
+        rows.append(np.repeat(d, 20)); cols.append(np.tile(d, 20)); vals.append(Mt.ravel())
+    M = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(f.n_field, f.n_field))
+    G = gr.discrete_gradient(m, f)
+    ref = np.asarray(G.multiply(M @ G).sum(axis=0)).ravel()
+    got = gr.nodal_volume_diagonal(m, eps)
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_nodal_surface_diagonal_is_gSg(mesh):
+    m, f = mesh
+    tri_ids = m.boundary_tris()[::3]
+    rows, cols, vals = [], [], []
+    for it in tri_ids:
+        S = robin.ned2_tri_stiff(m.nodes[:, m.tris[:, it]], 1.0)
+        d = f.tri_to_field[:, it]
+        rows.append(np.repeat(d, 8)); cols.append(np.tile(d, 8)); vals.append(np.real(S).ravel())
+    S = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(f.n_field, f.n_field))
+    G = gr.discrete_gradient(m, f)
+    ref = np.asarray(G.multiply(S @ G).sum(axis=0)).ravel()
+    got = gr.nodal_surface_diagonal(m, tri_ids)
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
