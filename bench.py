@@ -58,7 +58,7 @@ def cpu_baseline(n_tets_sample: int = 100_000) -> dict:
 
 
 def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method: str = "cocg",
-                idr_s: int = 4, restart: int = 20000) -> dict:
+                idr_s: int = 4, restart: int = 20000, precond: str = "aux") -> dict:
     """WR-90 waveguide (Kuhn 11x5x18, 5,940 tets, 41,118 DOFs), TE10 ports,
     frequency points from the 201-point 8-12 GHz grid, both ports excited
     per point.  Plain FP64 GPU path: the FP64 solver one (frequency, port)
@@ -74,7 +74,7 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     plain_f = grid[np.linspace(0, 200, n_plain).astype(int)]
     mixed_f = grid[np.linspace(0, 200, n_mixed).astype(int)]
     kw = dict(method=method, idr_s=idr_s, restart=restart, max_cycles=400 if method == "gmres" else 40,
-              keep_fields=False, inner_rel32={"gmres": 1e-5, "idr": 1e-4, "cocg": 1e-2}[method])
+              keep_fields=False, inner_rel32={"gmres": 1e-5, "idr": 1e-4, "cocg": 1e-2}[method], precond=precond)
     kw_m = dict(kw)
     if method == "idr":  # IDR(s) runs in FP64 only; the mixed path stays GMRES-IR
         kw_m.update(method="gmres", restart=100, max_cycles=400, inner_rel32=1e-5)
@@ -96,6 +96,8 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
         "workload": "WR-90 22.86x10.16x40 mm, Kuhn 11x5x18 (5,940 tets, 41,118 DOFs), TE10 ports, 8-12 GHz",
         "krylov": {"idr": f"IDR({idr_s})", "gmres": f"GMRES({restart})",
                    "cocg": "COCG on (A+A^T)/2 in FP64 refinement"}[method],
+        "preconditioner": ("Jacobi + auxiliary gradient space (G D_g^-1 G^T)" if method == "cocg" and precond == "aux"
+                           else "Jacobi"),
         "fp64_plain": {"points": len(plain_f), "converged": ok_p, "seconds": round(tp, 3),
                        "freq_points_per_s": round(ok_p / tp, 4),
                        "mean_iters_per_system": float(rp.iters.mean()), "max_relres": float(rp.relres.max())},
@@ -129,6 +131,94 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     return out
 
 
+def config_bench(name: str, nf: int, fblock: int) -> dict:
+    """Full-size solves of the other named workloads (one JSON object):
+
+    * cavity: PEC cavity 60x40x30 mm, Kuhn 48x32x24 (221,184 tets), dielectric
+      block, probe-fed lumped port, 5 GHz: mixed (complex64 inner, FP64
+      refinement) against FP64 inner solves -- |dS11| and field difference;
+    * patch: slab antenna in an air box with a first-order ABC (999,600
+      tets), `nf` of the 51 points over 2-3 GHz, FP64 inner solves in
+      blocks of `fblock`;
+    * family: 16 jittered WR-90 sections of 20k-120k DOFs, 16 points each
+      over 8-12 GHz, one context per mesh, solved from 16 host threads at
+      once and one after another."""
+    import threading
+    from emerge_amd import workloads as w
+    from emerge_amd.sweep import FrequencySweep
+    kw = dict(method="cocg", restart=40000, max_cycles=40, inner_rel32=1e-2, keep_fields=True)
+    out = {"config": name}
+    if name == "cavity":
+        p = w.cavity()
+        sw = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, w.cavity_bcs(p))
+        res = {}
+        for prec in ("mixed", "fp64"):
+            t0 = time.perf_counter()
+            r = sw.solve([5e9], precision=prec, fblock=1, **kw)
+            res[prec] = (r, time.perf_counter() - t0)
+            print(f"[bench] cavity {prec} {res[prec][1]:.2f} s relres {r.relres.max():.2e}", file=sys.stderr,
+                  flush=True)
+        (rm, tm), (rf, tf) = res["mixed"], res["fp64"]
+        a, b = rm.fields[(0, 1)], rf.fields[(0, 1)]
+        out.update(workload="cavity 60x40x30 mm, Kuhn 48x32x24 (221,184 tets), 5 GHz, lumped port",
+                   dofs=int(p.basis.n_field), reduced=int(sw.n_red),
+                   mixed={"seconds": round(tm, 3), "relres": float(rm.relres.max()), "iters": int(rm.iters.sum()),
+                          "escalated": rm.fp64_escalated, "S11": [float(rm.S[0, 0, 0].real), float(rm.S[0, 0, 0].imag)]},
+                   fp64={"seconds": round(tf, 3), "relres": float(rf.relres.max()), "iters": int(rf.iters.sum()),
+                         "S11": [float(rf.S[0, 0, 0].real), float(rf.S[0, 0, 0].imag)]},
+                   dS11=float(abs(rm.S[0, 0, 0] - rf.S[0, 0, 0])),
+                   field_rel_diff=float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    elif name == "patch":
+        p = w.patch_antenna()
+        sw = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, w.patch_bcs(p))
+        grid = np.linspace(2e9, 3e9, 51)
+        freqs = grid[np.round(np.linspace(0, 50, nf)).astype(int)]
+        kw["keep_fields"] = False
+        t0 = time.perf_counter()
+        r = sw.solve(freqs, precision="fp64", fblock=fblock, **kw)
+        dt = time.perf_counter() - t0
+        ok = int(np.sum(np.all(r.relres <= 1e-8, axis=1)))
+        out.update(workload="patch antenna slab 60x60x1.52 mm in an air box, first-order ABC, graded Kuhn "
+                            f"70x70x34 ({p.mesh.n_tets} tets)", dofs=int(p.basis.n_field), reduced=int(sw.n_red),
+                   precision="fp64", points=len(freqs), converged=ok, seconds=round(dt, 2),
+                   freq_points_per_s=round(ok / dt, 4), iters=r.iters.ravel().tolist(),
+                   relres_max=float(r.relres.max()),
+                   s11_db={f"{f / 1e9:.2f}": round(float(20 * np.log10(abs(r.S[i, 0, 0]))), 3)
+                           for i, f in enumerate(freqs)})
+    elif name == "family":
+        probs = w.waveguide_family()
+        sws = [FrequencySweep(q.mesh, q.basis, q.eps_r, q.inv_mu_r, w.waveguide_bcs(q)) for q in probs]
+        freqs = np.linspace(8e9, 12e9, nf)
+        kf = dict(kw, keep_fields=False, fblock=32, restart=20000)
+        for sw in sws[:2]:
+            sw.solve(freqs[:2], precision="mixed", **kf)
+        t0 = time.perf_counter()
+        seq = [sw.solve(freqs, precision="mixed", **kf) for sw in sws]
+        ts = time.perf_counter() - t0
+        conc = [None] * len(sws)
+
+        def run(i):
+            conc[i] = sws[i].solve(freqs, precision="mixed", **kf)
+
+        th = [threading.Thread(target=run, args=(i,)) for i in range(len(sws))]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        tc = time.perf_counter() - t0
+        ok_s = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in seq)
+        ok_c = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in conc)
+        out.update(workload="16 jittered WR-90 sections (Kuhn 11x5xnz), DOFs U[20k,120k] seed 0",
+                   dofs=[int(q.basis.n_field) for q in probs], points_per_mesh=len(freqs),
+                   sequential={"seconds": round(ts, 3), "converged": ok_s, "freq_points_per_s": round(ok_s / ts, 3)},
+                   concurrent={"seconds": round(tc, 3), "converged": ok_c, "freq_points_per_s": round(ok_c / tc, 3)},
+                   max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))))
+    else:
+        raise ValueError(name)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +235,11 @@ def main() -> int:
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=None,
                     help="GMRES dimension / IDR matvecs per cycle / COCG iterations per cycle")
+    ap.add_argument("--config", choices=["cavity", "patch", "family"], default=None,
+                    help="run one full-size solve workload instead of the default bench")
+    ap.add_argument("--config-points", type=int, default=None, help="frequency points of --config")
+    ap.add_argument("--precond", default="aux", choices=["aux", "jacobi"],
+                    help="COCG preconditioner: Jacobi + auxiliary gradient space, or Jacobi alone")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="counter calibration: only run 3 device-to-device copies of this many MiB")
     args = ap.parse_args()
@@ -172,9 +267,15 @@ def main() -> int:
     from emerge_amd.assembler import MeshOnDevice
     from emerge_amd.workloads import cube_assembly
 
+    if args.config:
+        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16}[args.config]
+        print(json.dumps(config_bench(args.config, nf, min(args.fblock, 8) if args.config == "patch" else 32)),
+              flush=True)
+        return 0
+
     if args.sweep_only:
         print(json.dumps({"sweep": sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock, 0, args.method,
-                                               args.idr_s, args.restart)}), flush=True)
+                                               args.idr_s, args.restart, args.precond)}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -226,7 +327,7 @@ def main() -> int:
     sweep = None
     if not args.no_sweep:
         sweep = sweep_bench(args.sweep_plain, args.sweep_mixed, args.fblock,
-                            0 if args.no_cpu_baseline else 2, args.method, args.idr_s, args.restart)
+                            0 if args.no_cpu_baseline else 2, args.method, args.idr_s, args.restart, args.precond)
         print(f"[bench] sweep {json.dumps(sweep)}", file=sys.stderr, flush=True)
 
     line = {

@@ -1277,22 +1277,37 @@ __global__ void k_aux_dg(Op<R> op, const double* __restrict__ k2, const double2*
     dginv[k] = m2 > 0 ? from_d<R>(make_double2(dr / m2, -di / m2)) : cmk<R>(R(0), R(0));
 }
 
-// t = D_g^-1 o (G^T r), layout [nodal][f]
+// t = D_g^-1 o (G^T r), layout [nodal][f].  A nodal row of G^T (a vertex
+// gathers from every DOF of its edge star, 60-100 entries) is split over
+// LPR lanes: F lanes per entry, G = LPR / F entries in flight, reduced by a
+// fixed-order butterfly; 64 / LPR rows per wavefront.
 template <typename R, int F>
 __global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restrict__ r,
                                                 const cplx<R>* __restrict__ dginv, cplx<R>* __restrict__ t) {
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= op.aux_n * F) return;
-    const int64_t j = k / F;
-    const int f = int(k % F);
+    constexpr int LPR = F >= 16 ? 64 : 16;
+    constexpr int G = LPR / F;
+    constexpr int RPW = 64 / LPR;
+    const int lane = threadIdx.x & 63;
+    const int l = lane % LPR;
+    const int f = l % F, g = l / F;
+    const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const int64_t j = wave * RPW + lane / LPR;
     R ax = R(0), ay = R(0);
-    for (int e = op.tptr[j]; e < op.tptr[j + 1]; ++e) {
-        const R w = op.tval[e];
-        const cplx<R> v = r[int64_t(op.tidx[e]) * F + f];
-        ax += w * v.x;
-        ay += w * v.y;
+    if (j < op.aux_n) {
+        const int e1 = op.tptr[j + 1];
+        for (int e = op.tptr[j] + g; e < e1; e += G) {
+            const R w = op.tval[e];
+            const cplx<R> v = r[int64_t(op.tidx[e]) * F + f];
+            ax += w * v.x;
+            ay += w * v.y;
+        }
     }
-    t[k] = cmul(dginv[k], cmk<R>(ax, ay));
+#pragma unroll
+    for (int off = LPR / 2; off >= F; off >>= 1) {
+        ax += __shfl_xor(ax, off);
+        ay += __shfl_xor(ay, off);
+    }
+    if (j < op.aux_n && g == 0) t[j * F + f] = cmul(dginv[j * F + f], cmk<R>(ax, ay));
 }
 
 // z = dinv o r + G t and per-workgroup partial sums of r^T z (same grid,
@@ -1406,12 +1421,15 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     const unsigned gb = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
     k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
     const unsigned gaux = aux ? grid_for(op.aux_n * F, 256) : 1;
+    // k_aux_gt: 64 / LPR nodal rows per wavefront, 4 wavefronts per workgroup
+    const int rpw_gt = F >= 16 ? 1 : 4;
+    const unsigned ggt = aux ? grid_for((op.aux_n + rpw_gt - 1) / rpw_gt, 4) : 1;
     if (aux) k_aux_dg<R, F><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv.p);
     EMG_HIP(c, hipGetLastError());
     // z = M r with M = D^-1 (+ G D_g^-1 G^T)
     auto precondition = [&]() {
         if (!aux) return;
-        k_aux_gt<R, F><<<gaux, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
+        k_aux_gt<R, F><<<ggt, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
         k_aux_z<R, F><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg.p, z.p, cs);
     };
     auto dots64 = [&](const double2* a, double2* out) {
