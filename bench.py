@@ -89,6 +89,12 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     t0 = time.perf_counter()
     rp = sw.solve(plain_f, precision="fp64", fblock=1, **kw)
     tp = time.perf_counter() - t0
+    tuning = None
+    if fblock == "auto":
+        fblock = sw.autotune_fblock(mixed_f, precision="mixed", method=kw_m["method"], precond=precond)
+        tuning = {str(k): round(v * 1e6, 3) for k, v in sw.tuning["seconds_per_system_iteration"].items()}
+        print(f"[bench] sweep: autotuned fblock {fblock} (us per system-iteration {tuning})", file=sys.stderr,
+              flush=True)
     print(f"[bench] sweep: fp64 plain {tp:.2f} s; mixed batched, {len(mixed_f)} points", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
@@ -108,6 +114,7 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
                        "mean_iters_per_system": float(rp.iters.mean()), "max_relres": float(rp.relres.max())},
         "mixed_batched": {"points": len(mixed_f), "converged": ok_m, "seconds": round(tm, 3),
                           "freq_points_per_s": round(ok_m / tm, 4), "fblock": fblock,
+                          "fblock_tuning_us_per_system_iteration": tuning,
                           "mean_iters_per_system": float(rm.iters.mean()), "max_relres": float(rm.relres.max())},
     }
     out["speedup_mixed_vs_fp64"] = round(out["mixed_batched"]["freq_points_per_s"]
@@ -233,7 +240,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-plain", type=int, default=4, help="FP64 plain-path frequency points")
     ap.add_argument("--sweep-mixed", type=int, default=201, help="mixed batched frequency points")
-    ap.add_argument("--fblock", type=int, default=32, help="systems per mixed batch")
+    ap.add_argument("--fblock", default="32", help="systems per mixed batch (1..32) or 'auto' (tuned per mesh)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-only", action="store_true", help="profile helper: run only the solve sweep")
     ap.add_argument("--method", default="cocg", choices=["gmres", "idr", "cocg"])
@@ -248,6 +255,7 @@ def main() -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="counter calibration: only run 3 device-to-device copies of this many MiB")
     args = ap.parse_args()
+    args.fblock = args.fblock if args.fblock == "auto" else int(args.fblock)
     if args.restart is None:
         args.restart = {"gmres": 100, "idr": 30000, "cocg": 20000}[args.method]
     if args.gpus != 1:
@@ -274,7 +282,8 @@ def main() -> int:
 
     if args.config:
         nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16}[args.config]
-        print(json.dumps(config_bench(args.config, nf, min(args.fblock, 8) if args.config == "patch" else 32)),
+        fb = 8 if args.fblock == "auto" else args.fblock
+        print(json.dumps(config_bench(args.config, nf, min(fb, 8) if args.config == "patch" else 32)),
               flush=True)
         return 0
 

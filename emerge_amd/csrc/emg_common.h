@@ -138,6 +138,7 @@ struct emg_ctx {
 
     // auxiliary gradient space of the COCG preconditioner (emg_aux_set)
     int64_t n_escalated = 0;      // systems of the last emg_sweep_solve finished with FP64 inner solves
+    bool escalate = true;         // continue stalled complex64 systems with complex128 inner solves
     int64_t aux_n = 0;            // nodal functions; 0 = Jacobi only
     int aux_nsurf = 0;
     emg::DBuf<int32_t> aux_gptr, aux_gidx, aux_tptr, aux_tidx;

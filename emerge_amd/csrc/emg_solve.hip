@@ -1633,6 +1633,12 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
 }
 }  // namespace
 
+extern "C" int emg_solver_set_escalation(emg_ctx* c, int on) {
+    EMG_CHECK_CTX(c);
+    c->escalate = on != 0;
+    return EMG_OK;
+}
+
 extern "C" int emg_sweep_escalated(emg_ctx* c, int64_t* n) {
     EMG_CHECK_CTX(c);
     if (!n) return fail(c, EMG_ERR_ARG, "emg_sweep_escalated: null output");
@@ -1800,7 +1806,7 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
                            ? dispatch_batch<float>(c, F, op32, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st)
                            : dispatch_batch<double>(c, F, op64s, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st);
         if (rc != EMG_OK) return rc;
-        if (precision == 1 && c->krylov_method == 2) {
+        if (precision == 1 && c->krylov_method == 2 && c->escalate) {
             // systems the complex64 inner solves could not finish continue with
             // complex128 inner solves from their current iterate (the
             // refinement recomputes b - A x in FP64; finished systems skip)

@@ -422,7 +422,50 @@ class FrequencySweep:
                 S[i] = apply_s_caps(S[i])
         return S
 
-    def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock: int = 16, restart: int = 100,
+    def autotune_fblock(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
+                        candidates=(4, 8, 16, 32), iters: int = 200, idr_s: int = 4,
+                        inner_rel32: float = 1e-2) -> int:
+        """Frequency-block width with the least device time per
+        system-iteration on this mesh: each candidate F solves the first F
+        (frequency, port) systems of `freqs` for at most `iters` inner
+        iterations (one refinement cycle, complex128 continuation off), timed
+        after a warm-up run; time is divided by the iterations actually run.
+        The result is cached per (precision, method, precond)."""
+        import time as _time
+        key = (precision, method, precond)
+        cache = self.__dict__.setdefault("_fblock_tuned", {})
+        if key in cache:
+            return cache[key]
+        freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+        k2, gam, coef = self.batch_coefficients(freqs)
+        nb = k2.shape[0]
+        self._set_aux(method == "cocg" and precond == "aux")
+        self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s), float(inner_rel32))
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        best, best_f = np.inf, 1
+        timings = {}
+        self.ctx.call("emg_solver_set_escalation", 0)
+        try:
+            for F in candidates:
+                rows = np.arange(F) % nb
+                run = dict(fblock=F, tol=1e-300, restart=iters if method != "gmres" else min(iters, 100),
+                           max_cycles=1, keep_fields=False)
+                args = (k2[rows], gam[rows] if gam.size else gam, coef[rows], prec)
+                self._run_rows(*args, **run)
+                t0 = _time.perf_counter()
+                _, _, it, _ = self._run_rows(*args, **run)
+                dt = _time.perf_counter() - t0
+                per = dt / max(int(it.sum()), 1)
+                timings[F] = per
+                if per < best:
+                    best, best_f = per, F
+        finally:
+            self.ctx.call("emg_solver_set_escalation", 1)
+        cache[key] = best_f
+        self.tuning = dict(cache=dict(cache), seconds_per_system_iteration=timings)
+        return best_f
+
+    def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock=16, restart: int = 100,
               max_cycles: int = 200, keep_fields: bool = True, method: str = "gmres", idr_s: int = 4,
               inner_rel32: float = 1e-5, precond: str = "aux", fallback: bool = True) -> SweepResult:
         """Solve every (frequency, active port) system.  method 'gmres' =
@@ -433,10 +476,14 @@ class FrequencySweep:
         only): 'aux' = Jacobi plus the auxiliary gradient-space correction
         (``aux_space``), 'jacobi' = Jacobi alone.  fallback (mixed only):
         systems that miss tol are solved again in FP64
-        (``SweepResult.fp64_fallback`` marks them)."""
+        (``SweepResult.fp64_fallback`` marks them).  fblock = systems per
+        device batch (1..32) or 'auto' (``autotune_fblock``)."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
         if precond not in ("aux", "jacobi"):
             raise ValueError("precond must be 'aux' or 'jacobi'")
+        if fblock == "auto":
+            fblock = self.autotune_fblock(freqs, precision=precision, method=method, precond=precond,
+                                          idr_s=idr_s, inner_rel32=inner_rel32)
         self._set_aux(method == "cocg" and precond == "aux")
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s),
                       float(inner_rel32))

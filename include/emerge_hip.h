@@ -217,6 +217,9 @@ int emg_solver_config(emg_ctx* ctx, int method, int idr_s, double inner_rel32);
  * inner solves from their current iterate.  No reference counterpart (the
  * reference solves in FP64 throughout). */
 int emg_sweep_escalated(emg_ctx* ctx, int64_t* n);
+/* on = 0 turns that continuation off (default on); used by the host's
+ * block-width autotuner, whose timing probes stop after a few iterations. */
+int emg_solver_set_escalation(emg_ctx* ctx, int on);
 
 /* Auxiliary gradient space of the COCG preconditioner (method 2 of
  * emg_sweep_solve): the inner solves apply z = D^-1 r + G D_g^-1 G^T r with

@@ -311,3 +311,19 @@ def test_uneven_meshes_solved_concurrently():
         assert np.array_equal(a.S, b.S)
         # |S21| -> 1 on the lossless guide above cutoff
         assert np.all(np.abs(np.abs(b.S[:, 1, 0]) - 1) <= 2e-2)
+
+
+def test_autotuned_block_width(sweep):
+    """fblock='auto' times each candidate width on this mesh, picks one of
+    them, and the tuned sweep still converges to the oracle's S."""
+    freqs = np.linspace(8e9, 12e9, 12)
+    f = sweep.autotune_fblock(freqs, candidates=(4, 8, 16))
+    assert f in (4, 8, 16)
+    assert set(sweep.tuning["seconds_per_system_iteration"]) == {4, 8, 16}
+    res = sweep.solve(freqs[:3], precision="mixed", tol=1e-8, fblock="auto", restart=20000, max_cycles=30,
+                      method="cocg", inner_rel32=1e-2)
+    assert np.all(res.relres <= 1e-8)
+    for i, fr in enumerate(freqs[:3]):
+        o = wg_case.oracle_solution(float(fr))
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6
+    print(f"autotuned fblock={f} timings={sweep.tuning}")
