@@ -115,6 +115,7 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
         "mixed_batched": {"points": len(mixed_f), "converged": ok_m, "seconds": round(tm, 3),
                           "freq_points_per_s": round(ok_m / tm, 4), "fblock": fblock,
                           "fblock_tuning_us_per_system_iteration": tuning,
+                          "fp64_escalated_systems": int(rm.fp64_escalated),
                           "mean_iters_per_system": float(rm.iters.mean()), "max_relres": float(rm.relres.max())},
     }
     out["speedup_mixed_vs_fp64"] = round(out["mixed_batched"]["freq_points_per_s"]

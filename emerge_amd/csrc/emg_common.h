@@ -139,6 +139,9 @@ struct emg_ctx {
     // auxiliary gradient space of the COCG preconditioner (emg_aux_set)
     int64_t n_escalated = 0;      // systems of the last emg_sweep_solve finished with FP64 inner solves
     bool escalate = true;         // continue stalled complex64 systems with complex128 inner solves
+    bool safe32 = false;          // complex64 COCG with FP64 real stiffness and FP64 accumulation
+    int er64_state = 0;           // 0 unknown, 1 reduced E is real (E64r holds it), 2 complex
+    emg::DBuf<double> E64r;       // nnz_red: real part of the reduced E values
     int64_t aux_n = 0;            // nodal functions; 0 = Jacobi only
     int aux_nsurf = 0;
     emg::DBuf<int32_t> aux_gptr, aux_gidx, aux_tptr, aux_tidx;

@@ -53,6 +53,12 @@ struct Op {
     const R* tval = nullptr;
     const double2* dvol = nullptr;
     const double* dsurf = nullptr;  // [aux_nsurf][aux_n]
+    // complex64 COCG only: the stiffness values in FP64 when they are real
+    // (real mu_r); the SpMV then accumulates in FP64
+    const double* Er64 = nullptr;
+    // with Er64: the gradient-space correction also runs in FP64 (G values)
+    const double* gval_d = nullptr;
+    const double* tval_d = nullptr;
 };
 
 inline unsigned grid_for(int64_t n, int block) { return unsigned((n + block - 1) / block); }
@@ -1002,11 +1008,15 @@ __device__ __forceinline__ void load_pair(const double2* p, double2& a, double2&
 // (ar + i ai)[v] += (E[j] - k2[v] B[j]) z[idx[j]][f0 + v] for the V systems
 // of a lane (real and imaginary parts in separate arrays: they stay in
 // registers for complex128 too)
-template <typename R, int F, int V>
+template <typename R, int F, int V, typename A>
 __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restrict__ z, const R (&kk)[V], int f0,
-                                         int j, R (&ar)[V], R (&ai)[V]) {
+                                         int j, A (&ar)[V], A (&ai)[V]) {
     const int64_t c = op.idx[j];
-    const cplx<R> e = op.E[j];
+    // matrix values are loaded before the gathers so that all are in flight
+    cplx<R> e = cmk<R>(R(0), R(0));
+    double er = 0.0;
+    if constexpr (sizeof(A) > sizeof(R)) er = op.Er64[j];
+    else e = op.E[j];
     const cplx<R> b = op.B ? op.B[j] : cmk<R>(R(0), R(0));
     const cplx<R>* zc = z + c * F + f0;
     cplx<R> zv[V];
@@ -1017,11 +1027,23 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
 #pragma unroll
         for (int v = 0; v < V; ++v) zv[v] = zc[v];
     }
+    if constexpr (sizeof(A) > sizeof(R)) {
+        // real FP64 stiffness, FP64 accumulation: E annihilates gradients, and
+        // rounding E to FP32 would leave eps32 |E| |z| in rows that cancel
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-        const R mr = e.x - kk[v] * b.x, mi = e.y - kk[v] * b.y;
-        ar[v] += mr * zv[v].x - mi * zv[v].y;
-        ai[v] += mr * zv[v].y + mi * zv[v].x;
+        for (int v = 0; v < V; ++v) {
+            const double mr = er - double(kk[v]) * double(b.x), mi = -double(kk[v]) * double(b.y);
+            ar[v] += mr * double(zv[v].x) - mi * double(zv[v].y);
+            ai[v] += mr * double(zv[v].y) + mi * double(zv[v].x);
+        }
+    } else {
+        (void)er;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const R mr = e.x - kk[v] * b.x, mi = e.y - kk[v] * b.y;
+            ar[v] += mr * zv[v].x - mi * zv[v].y;
+            ai[v] += mr * zv[v].y + mi * zv[v].x;
+        }
     }
 }
 
@@ -1034,7 +1056,7 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
 // reduction leaves each row sum in all G lanes of its systems, so the row
 // end (surface terms, q store, p^T q) is spread one system per lane, and
 // those lanes load their p before the gathers.
-template <typename R, int F>
+template <typename R, int F, typename A = R>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
                                                         const cplx<R>* __restrict__ gam,
                                                         const cplx<R>* __restrict__ p, cplx<R>* __restrict__ q,
@@ -1068,22 +1090,22 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
             px = pv.x;
             py = pv.y;
         }
-        R ar[V], ai[V], br[V], bi[V];
+        A ar[V], ai[V], br[V], bi[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) ar[v] = ai[v] = br[v] = bi[v] = R(0);
+        for (int v = 0; v < V; ++v) ar[v] = ai[v] = br[v] = bi[v] = A(0);
         const int j1 = op.ptr[row + 1];
         int j = op.ptr[row] + g;
         for (; j + G < j1; j += 2 * G) {
-            cg_entry<R, F, V>(op, p, kk, f0, j, ar, ai);
-            cg_entry<R, F, V>(op, p, kk, f0, j + G, br, bi);
+            cg_entry<R, F, V, A>(op, p, kk, f0, j, ar, ai);
+            cg_entry<R, F, V, A>(op, p, kk, f0, j + G, br, bi);
         }
-        if (j < j1) cg_entry<R, F, V>(op, p, kk, f0, j, ar, ai);
+        if (j < j1) cg_entry<R, F, V, A>(op, p, kk, f0, j, ar, ai);
         // the pick is done on scalars: a select of vector-type structs is
         // lowered to an indexed stack load for complex128
-        R ax = R(0), ay = R(0);
+        A ax = A(0), ay = A(0);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            R tx = ar[v] + br[v], ty = ai[v] + bi[v];
+            A tx = ar[v] + br[v], ty = ai[v] + bi[v];
 #pragma unroll
             for (int off = 32; off >= FG; off >>= 1) {
                 tx += __shfl_xor(tx, off);
@@ -1092,7 +1114,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
             ax = (v == g) ? tx : ax;
             ay = (v == g) ? ty : ay;
         }
-        cplx<R> a = cmk<R>(ax, ay);
+        cplx<R> a = cmk<R>(R(ax), R(ay));
         if (own) {
             if (op.sptr) {
                 for (int k = op.sptr[row]; k < op.sptr[row + 1]; ++k) {
@@ -1259,9 +1281,9 @@ __global__ void k_cg_accum(int64_t nq, const cplx<R>* __restrict__ dx, const dou
 }
 
 // D_g^-1 per nodal function and system: 1 / (-k2[f] dvol[j] + sum_s gam[s][f] dsurf[s][j])
-template <typename R, int F>
+template <typename R, int F, typename A = R>
 __global__ void k_aux_dg(Op<R> op, const double* __restrict__ k2, const double2* __restrict__ gam,
-                         cplx<R>* __restrict__ dginv) {
+                         cplx<A>* __restrict__ dginv) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= op.aux_n * F) return;
     const int64_t j = k / F;
@@ -1274,16 +1296,16 @@ __global__ void k_aux_dg(Op<R> op, const double* __restrict__ k2, const double2*
         di += gam[sb * F + f].y * w;
     }
     const double m2 = dr * dr + di * di;
-    dginv[k] = m2 > 0 ? from_d<R>(make_double2(dr / m2, -di / m2)) : cmk<R>(R(0), R(0));
+    dginv[k] = m2 > 0 ? from_d<A>(make_double2(dr / m2, -di / m2)) : cmk<A>(A(0), A(0));
 }
 
 // t = D_g^-1 o (G^T r), layout [nodal][f].  A nodal row of G^T (a vertex
 // gathers from every DOF of its edge star, 60-100 entries) is split over
 // LPR lanes: F lanes per entry, G = LPR / F entries in flight, reduced by a
 // fixed-order butterfly; 64 / LPR rows per wavefront.
-template <typename R, int F>
+template <typename R, int F, typename A = R>
 __global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restrict__ r,
-                                                const cplx<R>* __restrict__ dginv, cplx<R>* __restrict__ t) {
+                                                const cplx<A>* __restrict__ dginv, cplx<A>* __restrict__ t) {
     constexpr int LPR = F >= 16 ? 64 : 16;
     constexpr int G = LPR / F;
     constexpr int RPW = 64 / LPR;
@@ -1292,14 +1314,16 @@ __global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restr
     const int f = l % F, g = l / F;
     const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
     const int64_t j = wave * RPW + lane / LPR;
-    R ax = R(0), ay = R(0);
+    A ax = A(0), ay = A(0);
     if (j < op.aux_n) {
         const int e1 = op.tptr[j + 1];
         for (int e = op.tptr[j] + g; e < e1; e += G) {
-            const R w = op.tval[e];
+            A w;
+            if constexpr (sizeof(A) > sizeof(R)) w = op.tval_d[e];
+            else w = op.tval[e];
             const cplx<R> v = r[int64_t(op.tidx[e]) * F + f];
-            ax += w * v.x;
-            ay += w * v.y;
+            ax += w * A(v.x);
+            ay += w * A(v.y);
         }
     }
 #pragma unroll
@@ -1307,14 +1331,14 @@ __global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restr
         ax += __shfl_xor(ax, off);
         ay += __shfl_xor(ay, off);
     }
-    if (j < op.aux_n && g == 0) t[j * F + f] = cmul(dginv[j * F + f], cmk<R>(ax, ay));
+    if (j < op.aux_n && g == 0) t[j * F + f] = cmul(dginv[j * F + f], cmk<A>(ax, ay));
 }
 
 // z = dinv o r + G t and per-workgroup partial sums of r^T z (same grid,
 // stride and partial layout as k_cg_update, which leaves the ||r||^2 half)
-template <typename R, int F>
+template <typename R, int F, typename A = R>
 __global__ __launch_bounds__(kCgThreads) void k_aux_z(int64_t nq, Op<R> op, const cplx<R>* __restrict__ dinv,
-                                                      const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ t,
+                                                      const cplx<R>* __restrict__ r, const cplx<A>* __restrict__ t,
                                                       cplx<R>* __restrict__ z, CgState st) {
     __shared__ double2 red[kCgThreads];
     const int tid = threadIdx.x;
@@ -1323,13 +1347,17 @@ __global__ __launch_bounds__(kCgThreads) void k_aux_z(int64_t nq, Op<R> op, cons
         const int64_t row = k / F;
         const int f = int(k % F);
         const cplx<R> rv = r[k];
-        cplx<R> zv = cmul(dinv[k], rv);
+        const cplx<R> dz = cmul(dinv[k], rv);
+        A zx = A(dz.x), zy = A(dz.y);
         for (int e = op.gptr[row]; e < op.gptr[row + 1]; ++e) {
-            const R w = op.gval[e];
-            const cplx<R> v = t[int64_t(op.gidx[e]) * F + f];
-            zv.x += w * v.x;
-            zv.y += w * v.y;
+            A w;
+            if constexpr (sizeof(A) > sizeof(R)) w = op.gval_d[e];
+            else w = op.gval[e];
+            const cplx<A> v = t[int64_t(op.gidx[e]) * F + f];
+            zx += w * v.x;
+            zy += w * v.y;
         }
+        const cplx<R> zv = cmk<R>(R(zx), R(zy));
         z[k] = zv;
         const double2 d = to_d(cmul(rv, zv));
         rz.x += d.x;
@@ -1374,10 +1402,18 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     DBuf<double2> gam64;
     for (DBuf<cplx<R>>* b : {&pv, &q, &r, &z, &dx, &dinv}) EMG_HIP(c, b->alloc(nq));
     const bool aux = op.aux_n > 0;
+    // complex64 with an FP64 real stiffness: SpMV and the gradient-space
+    // correction accumulate in FP64 (the correction's t kept in FP64)
+    const bool safe = sizeof(R) < sizeof(double) && op.Er64 != nullptr;
     DBuf<cplx<R>> tg, dginv;
-    if (aux) {
+    DBuf<double2> tg64, dginv64;
+    if (aux && !safe) {
         EMG_HIP(c, tg.alloc(op.aux_n * F));
         EMG_HIP(c, dginv.alloc(op.aux_n * F));
+    }
+    if (aux && safe) {
+        EMG_HIP(c, tg64.alloc(op.aux_n * F));
+        EMG_HIP(c, dginv64.alloc(op.aux_n * F));
     }
     EMG_HIP(c, r64.alloc(nq));
     EMG_HIP(c, t64.alloc(nq));
@@ -1424,13 +1460,19 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     // k_aux_gt: 64 / LPR nodal rows per wavefront, 4 wavefronts per workgroup
     const int rpw_gt = F >= 16 ? 1 : 4;
     const unsigned ggt = aux ? grid_for((op.aux_n + rpw_gt - 1) / rpw_gt, 4) : 1;
-    if (aux) k_aux_dg<R, F><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv.p);
+    if (aux && !safe) k_aux_dg<R, F><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv.p);
+    if (aux && safe) k_aux_dg<R, F, double><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv64.p);
     EMG_HIP(c, hipGetLastError());
     // z = M r with M = D^-1 (+ G D_g^-1 G^T)
     auto precondition = [&]() {
         if (!aux) return;
-        k_aux_gt<R, F><<<ggt, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
-        k_aux_z<R, F><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg.p, z.p, cs);
+        if (safe) {
+            k_aux_gt<R, F, double><<<ggt, 256, 0, s>>>(op, r.p, dginv64.p, tg64.p);
+            k_aux_z<R, F, double><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg64.p, z.p, cs);
+        } else {
+            k_aux_gt<R, F><<<ggt, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
+            k_aux_z<R, F><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg.p, z.p, cs);
+        }
     };
     auto dots64 = [&](const double2* a, double2* out) {
         k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, a, partial.p);
@@ -1529,7 +1571,10 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         auto iterations = [&]() {
             for (int i = 0; i < blk; ++i) {
                 k_cg_pupd<R, F><<<gq, 256, 0, s>>>(nq, z.p, pv.p, cs);
-                k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, pv.p, q.p, cs);
+                if (safe)
+                    k_cg_spmv<R, F, double><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, pv.p, q.p, cs);
+                else
+                    k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, pv.p, q.p, cs);
                 k_cg_alpha<F><<<1, kCgThreads, 0, s>>>(int(ga), cs);
                 k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0, int(aux));
                 precondition();
@@ -1550,7 +1595,13 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         int it = 0;
         std::vector<int> active(F), it_best(F, 0);
         std::vector<double> best(F, 1.0);
-        for (int f = 0; f < F; ++f) active[f] = !(st.relres[f] <= P.tol);
+        // systems with an inner target this cycle (finished and given-up
+        // ones got a target they already meet)
+        for (int f = 0; f < F; ++f) active[f] = ht[f] < 1.0;
+        // a block runs until its slowest system stops: once the first system
+        // has converged at iteration k1, systems still running past
+        // max(4 k1, k1 + 400) are stopped and restarted by the refinement
+        int first_done = -1;
         while (it < P.restart) {
             if (use_graph) EMG_HIP(c, hipGraphLaunch(graph, s));
             else iterations();
@@ -1561,7 +1612,15 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             EMG_HIP(c, hipMemcpyAsync(hrr, rr.p, sizeof(double) * F, hipMemcpyDeviceToHost, s));
             EMG_HIP(c, hipStreamSynchronize(s));
             bool fin = true, stop = false;
+            if (first_done < 0)
+                for (int f = 0; f < F; ++f)
+                    if (active[f] && hdone[f]) first_done = it;
+            const bool cut = first_done >= 0 && it > std::max(4 * first_done, first_done + 400);
             for (int f = 0; f < F; ++f) {
+                if (cut && !hdone[f] && active[f]) {
+                    hdone[f] = 1;
+                    stop = true;
+                }
                 if (!hdone[f] && active[f]) {
                     if (hrr[f] < 0.25 * best[f]) {
                         best[f] = hrr[f];
@@ -1627,11 +1686,27 @@ int pow2_block(int64_t remaining, int fblock) {
 }  // namespace
 
 namespace {
+// real parts of complex values; flag set when any imaginary part is nonzero
+__global__ void k_real_part(int64_t n, const double2* __restrict__ v, double* __restrict__ re,
+                            int* __restrict__ nonreal) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double2 x = v[k];
+    re[k] = x.x;
+    if (x.y != 0.0) *nonreal = 1;
+}
+
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < n) b[k] = float(a[k]);
 }
 }  // namespace
+
+extern "C" int emg_solver_set_safe32(emg_ctx* c, int on) {
+    EMG_CHECK_CTX(c);
+    c->safe32 = on != 0;
+    return EMG_OK;
+}
 
 extern "C" int emg_solver_set_escalation(emg_ctx* c, int on) {
     EMG_CHECK_CTX(c);
@@ -1757,6 +1832,25 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         }
         op32.B = reinterpret_cast<const float2*>(c->Bs32.p);
     }
+    if (c->krylov_method == 2 && precision == 1 && c->safe32) {
+        // a real reduced stiffness (real mu_r) is kept in FP64 for the
+        // complex64 COCG SpMV (same 8 bytes per entry as complex64)
+        if (c->er64_state == 0) {
+            DBuf<int> flag;
+            EMG_HIP(c, flag.alloc(1));
+            EMG_HIP(c, hipMemsetAsync(flag.p, 0, sizeof(int), s));
+            EMG_HIP(c, c->E64r.alloc(std::max<int64_t>(c->nnz_red, 1)));
+            k_real_part<<<grid_for(c->nnz_red, 256), 256, 0, s>>>(
+                c->nnz_red, reinterpret_cast<const double2*>(c->E64.p), c->E64r.p, flag.p);
+            EMG_HIP(c, hipGetLastError());
+            int hf = 0;
+            EMG_HIP(c, hipMemcpyAsync(&hf, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipStreamSynchronize(s));
+            c->er64_state = hf ? 2 : 1;
+            if (hf) c->E64r.release();
+        }
+        if (c->er64_state == 1 && c->safe32) op32.Er64 = c->E64r.p;
+    }
     Op<double> op64s = op64;
     if (c->krylov_method == 2) op64s.B = reinterpret_cast<const double2*>(c->Bs64.p);
     if (c->krylov_method == 2 && c->aux_n > 0) {
@@ -1777,6 +1871,8 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         };
         wire(op32, c->aux_gval32.p, c->aux_tval32.p);
         wire(op64s, c->aux_gval64.p, c->aux_tval64.p);
+        op32.gval_d = c->aux_gval64.p;
+        op32.tval_d = c->aux_tval64.p;
     }
     SolveParams P;
     P.tol = tol;

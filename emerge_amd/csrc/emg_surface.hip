@@ -422,6 +422,7 @@ extern "C" int emg_system_reduce(emg_ctx* c, int64_t n_keep, const int32_t* keep
     c->have_system = true;
     c->have_bsym = false;
     c->aux_n = 0;  // the auxiliary space refers to the old reduced rows
+    c->er64_state = 0;
     c->have_surface = false;
     return EMG_OK;
 }

@@ -422,6 +422,14 @@ class FrequencySweep:
                 S[i] = apply_s_caps(S[i])
         return S
 
+    def needs_safe32(self, freqs) -> bool:
+        """True when (k0_min h_min)^2 < 0.05: on near-gradient fields E z
+        cancels to O(k0^2 h^2) of its terms, so FP32 stiffness values (relative
+        error 6e-8) would perturb the operator by 6e-8 / (k0 h)^2 there."""
+        k0 = 2 * np.pi * float(np.min(freqs)) / C0
+        h = float(np.min(self.mesh.edge_lengths))
+        return (k0 * h) ** 2 < 0.05
+
     def autotune_fblock(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
                         candidates=(4, 8, 16, 32), iters: int = 200, idr_s: int = 4,
                         inner_rel32: float = 1e-2) -> int:
@@ -467,7 +475,8 @@ class FrequencySweep:
 
     def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock=16, restart: int = 100,
               max_cycles: int = 200, keep_fields: bool = True, method: str = "gmres", idr_s: int = 4,
-              inner_rel32: float = 1e-5, precond: str = "aux", fallback: bool = True) -> SweepResult:
+              inner_rel32: float = 1e-5, precond: str = "aux", fallback: bool = True,
+              safe32="auto") -> SweepResult:
         """Solve every (frequency, active port) system.  method 'gmres' =
         GMRES(restart); 'idr' = IDR(idr_s) with a budget of `restart`
         matvecs per FP64 refinement cycle; 'cocg' = COCG on the symmetric
@@ -477,10 +486,16 @@ class FrequencySweep:
         (``aux_space``), 'jacobi' = Jacobi alone.  fallback (mixed only):
         systems that miss tol are solved again in FP64
         (``SweepResult.fp64_fallback`` marks them).  fblock = systems per
-        device batch (1..32) or 'auto' (``autotune_fblock``)."""
+        device batch (1..32) or 'auto' (``autotune_fblock``).  safe32
+        (mixed COCG): keep the real stiffness in FP64 and accumulate the
+        SpMV and the gradient-space correction in FP64; 'auto' turns it on
+        when ``needs_safe32`` says the mesh is fine against the wavelength."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
         if precond not in ("aux", "jacobi"):
             raise ValueError("precond must be 'aux' or 'jacobi'")
+        if safe32 == "auto":
+            safe32 = self.needs_safe32(freqs)
+        self.ctx.call("emg_solver_set_safe32", int(bool(safe32)))
         if fblock == "auto":
             fblock = self.autotune_fblock(freqs, precision=precision, method=method, precond=precond,
                                           idr_s=idr_s, inner_rel32=inner_rel32)

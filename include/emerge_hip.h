@@ -220,6 +220,13 @@ int emg_sweep_escalated(emg_ctx* ctx, int64_t* n);
 /* on = 0 turns that continuation off (default on); used by the host's
  * block-width autotuner, whose timing probes stop after a few iterations. */
 int emg_solver_set_escalation(emg_ctx* ctx, int on);
+/* on = 1: the complex64 COCG path keeps a real reduced stiffness in FP64 (8
+ * bytes per entry, as complex64) and accumulates its SpMV and the
+ * gradient-space correction in FP64; complex64 vectors otherwise.  For
+ * meshes with k0^2 h^2 << 1, where E z cancels to O(k0^2 h^2) of its terms
+ * on near-gradient z and FP32 stiffness values would swamp the result.  No
+ * reference counterpart (the reference solves in FP64). */
+int emg_solver_set_safe32(emg_ctx* ctx, int on);
 
 /* Auxiliary gradient space of the COCG preconditioner (method 2 of
  * emg_sweep_solve): the inner solves apply z = D^-1 r + G D_g^-1 G^T r with

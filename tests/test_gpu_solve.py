@@ -327,3 +327,17 @@ def test_autotuned_block_width(sweep):
         o = wg_case.oracle_solution(float(fr))
         assert np.abs(res.S[i] - o["S"]).max() <= 1e-6
     print(f"autotuned fblock={f} timings={sweep.tuning}")
+
+
+@pytest.mark.parametrize("safe32", [True, False])
+def test_mixed_safe32_modes_match_superlu(sweep, safe32):
+    """Complex64 COCG with the FP64 real stiffness and FP64 accumulation
+    (safe32) and without it: both converge to the oracle's S."""
+    freqs = np.array([8e9, 12e9])
+    res = sweep.solve(freqs, precision="mixed", tol=1e-8, fblock=4, restart=20000, max_cycles=30,
+                      method="cocg", inner_rel32=1e-2, safe32=safe32)
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(freqs):
+        o = wg_case.oracle_solution(float(f))
+        assert np.abs(res.S[i] - o["S"]).max() <= 1e-6
+    print(f"safe32={safe32} iters={res.iters.tolist()} escalated={res.fp64_escalated}")
