@@ -1016,6 +1016,7 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
     cplx<R> e = cmk<R>(R(0), R(0));
     double er = 0.0;
     if constexpr (sizeof(A) > sizeof(R)) er = op.Er64[j];
+    else if (sizeof(R) == sizeof(double) && op.Er64) e = cmk<R>(R(op.Er64[j]), R(0));
     else e = op.E[j];
     const cplx<R> b = op.B ? op.B[j] : cmk<R>(R(0), R(0));
     const cplx<R>* zc = z + c * F + f0;
@@ -1050,12 +1051,16 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
 // q = A_s p and per-workgroup partial sums of p^T q.  q is formed from p
 // every iteration (not by the recurrence q = A z + beta q, whose drift from
 // A p breaks complex64 COCG on strongly indefinite systems).
-// One wavefront per row; a lane covers V consecutive systems of the batch
-// (V/2 16-byte gathers per matrix entry) and every G-th entry of the row,
-// two entries per step so that their gathers overlap.  The butterfly
-// reduction leaves each row sum in all G lanes of its systems, so the row
-// end (surface terms, q store, p^T q) is spread one system per lane, and
-// those lanes load their p before the gathers.
+// A lane covers V consecutive systems of the batch (V/2 16-byte gathers per
+// matrix entry) and every GR-th entry of its row, two entries per step so
+// that their gathers overlap.  A row gets LPR = FG * GR lanes with GR = 16
+// entries in flight (rows hold ~45 entries), so small batches put several
+// rows in one wavefront instead of leaving lanes idle.  The butterfly
+// reduction stays inside the row's lanes and leaves the row sum in all GR
+// lanes of its systems, so the row end (surface terms, q store, p^T q) is
+// spread one system per lane, and those lanes load their p before the
+// gathers.  Control flow is uniform: every wavefront walks the same number
+// of row groups and lanes past the last row only add zeros.
 template <typename R, int F, typename A = R>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
                                                         const cplx<R>* __restrict__ gam,
@@ -1064,11 +1069,15 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     constexpr int VM = sizeof(R) == 4 ? 8 : 4;  // 8 complex64 or 4 complex128: 64 bytes per lane
     constexpr int V = F >= VM ? VM : F;         // systems per lane
     constexpr int FG = F / V;                   // lanes per entry
-    constexpr int G = 64 / FG;                  // entries in flight per wavefront (G >= V)
+    constexpr int GW = 64 / FG;                 // entry lanes per wavefront
+    constexpr int GR = GW >= 16 ? 16 : GW;      // entries in flight per row (GR >= V)
+    constexpr int LPR = FG * GR;                // lanes per row
+    constexpr int RPW = 64 / LPR;               // rows per wavefront
     constexpr int W = kCgThreads / 64;
     __shared__ double2 wsum[W][F];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int fg = lane % FG, g = lane / FG, f0 = fg * V;
+    const int rs = lane / LPR, l = lane % LPR;
+    const int fg = l % FG, g = l / FG, f0 = fg * V;
     const bool own = g < V;                     // row end of system f0 + g
     const int fo = f0 + (own ? g : 0);
     R kk[V];
@@ -1082,10 +1091,12 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     const int64_t xcd = blockIdx.x % 8;
     const int64_t r0 = xcd * per, r1 = min(op.n, r0 + per);
     const int64_t nw = int64_t(gridDim.x / 8) * W;
-    for (int64_t row = r0 + int64_t(blockIdx.x / 8) * W + wv; row < r1; row += nw) {
+    for (int64_t base = r0 + (int64_t(blockIdx.x / 8) * W + wv) * RPW; base < r1; base += nw * RPW) {
+        const int64_t row = base + rs;
+        const bool valid = row < r1;
         const int64_t o = row * F + fo;
         R px = R(0), py = R(0);
-        if (own) {
+        if (own && valid) {
             const cplx<R> pv = p[o];
             px = pv.x;
             py = pv.y;
@@ -1093,11 +1104,11 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
         A ar[V], ai[V], br[V], bi[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) ar[v] = ai[v] = br[v] = bi[v] = A(0);
-        const int j1 = op.ptr[row + 1];
-        int j = op.ptr[row] + g;
-        for (; j + G < j1; j += 2 * G) {
+        const int j1 = valid ? op.ptr[row + 1] : 0;
+        int j = valid ? op.ptr[row] + g : 0;
+        for (; j + GR < j1; j += 2 * GR) {
             cg_entry<R, F, V, A>(op, p, kk, f0, j, ar, ai);
-            cg_entry<R, F, V, A>(op, p, kk, f0, j + G, br, bi);
+            cg_entry<R, F, V, A>(op, p, kk, f0, j + GR, br, bi);
         }
         if (j < j1) cg_entry<R, F, V, A>(op, p, kk, f0, j, ar, ai);
         // the pick is done on scalars: a select of vector-type structs is
@@ -1107,7 +1118,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
         for (int v = 0; v < V; ++v) {
             A tx = ar[v] + br[v], ty = ai[v] + bi[v];
 #pragma unroll
-            for (int off = 32; off >= FG; off >>= 1) {
+            for (int off = LPR / 2; off >= FG; off >>= 1) {
                 tx += __shfl_xor(tx, off);
                 ty += __shfl_xor(ty, off);
             }
@@ -1115,7 +1126,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
             ay = (v == g) ? ty : ay;
         }
         cplx<R> a = cmk<R>(R(ax), R(ay));
-        if (own) {
+        if (own && valid) {
             if (op.sptr) {
                 for (int k = op.sptr[row]; k < op.sptr[row + 1]; ++k) {
                     const cplx<R> gv = gam[op.sbc[k] * F + fo];
@@ -1129,7 +1140,13 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
             dot.y += t.y;
         }
     }
-    if (own) wsum[wv][fo] = dot;
+    // the row slots of a wavefront hold partial sums of the same systems
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1) {
+        dot.x += __shfl_xor(dot.x, off);
+        dot.y += __shfl_xor(dot.y, off);
+    }
+    if (own && rs == 0) wsum[wv][fo] = dot;
     __syncthreads();
     if (threadIdx.x < F) {
         double2 sum = make_double2(0.0, 0.0);
@@ -1832,7 +1849,7 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         }
         op32.B = reinterpret_cast<const float2*>(c->Bs32.p);
     }
-    if (c->krylov_method == 2 && precision == 1 && c->safe32) {
+    if (c->krylov_method == 2) {
         // a real reduced stiffness (real mu_r) is kept in FP64 for the
         // complex64 COCG SpMV (same 8 bytes per entry as complex64)
         if (c->er64_state == 0) {
@@ -1853,6 +1870,9 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     }
     Op<double> op64s = op64;
     if (c->krylov_method == 2) op64s.B = reinterpret_cast<const double2*>(c->Bs64.p);
+    // complex128 COCG reads the real stiffness as 8-byte reals (of 36 bytes
+    // per entry, 28 remain)
+    if (c->krylov_method == 2 && c->er64_state == 1) op64s.Er64 = c->E64r.p;
     if (c->krylov_method == 2 && c->aux_n > 0) {
         if (c->aux_nsurf != nbc)
             return fail(c, EMG_ERR_STATE, "emg_sweep_solve: auxiliary space has %d surface diagonals, system has %d",
