@@ -151,8 +151,8 @@ def config_bench(name: str, nf: int, fblock: int) -> dict:
       block, probe-fed lumped port, 5 GHz: mixed (complex64 inner, FP64
       refinement) against FP64 inner solves -- |dS11| and field difference;
     * patch: slab antenna in an air box with a first-order ABC (999,600
-      tets), `nf` of the 51 points over 2-3 GHz, FP64 inner solves in
-      blocks of `fblock`;
+      tets), `nf` of the 51 points over 2-3 GHz (spread over the band),
+      FP64 inner solves in blocks of `fblock`;
     * family: 16 jittered WR-90 sections of 20k-120k DOFs, 16 points each
       over 8-12 GHz, one context per mesh, solved from 16 host threads at
       once and one after another."""
@@ -284,7 +284,7 @@ def main() -> int:
     if args.config:
         nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16}[args.config]
         fb = 8 if args.fblock == "auto" else args.fblock
-        print(json.dumps(config_bench(args.config, nf, min(fb, 8) if args.config == "patch" else 32)),
+        print(json.dumps(config_bench(args.config, nf, fb if args.config == "patch" else 32)),
               flush=True)
         return 0
 

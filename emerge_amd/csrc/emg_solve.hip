@@ -1393,16 +1393,39 @@ __global__ __launch_bounds__(kCgThreads) void k_aux_z(int64_t nq, Op<R> op, cons
     if (tid < F) st.part[int64_t(blockIdx.x) * 2 * F + tid] = red[tid];
 }
 
-// undo the last refinement step of the flagged systems: x64 -= ||r64_prev|| dx
+// minimal-residual refinement step: with d = r_old - r_new = A (s dx) of the
+// last correction s dx (s = ||r_old||), take x += (w - 1) s dx and
+// r = r_old - w d, w = d^H r_old / d^H d per system, so that no step
+// increases ||r|| (w = 1 is the plain refinement step)
+template <typename R, int F>
+__global__ void k_mr_apply(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm_prev,
+                           const double2* __restrict__ om, double2* __restrict__ x,
+                           const double2* __restrict__ r_old, const double2* __restrict__ d,
+                           double2* __restrict__ r) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const int f = int(k % F);
+    const double2 w = om[f];
+    const double sc = sqrt(fmax(nrm_prev[f].x, 0.0));
+    const double2 s = to_d(dx[k]);
+    const double ar = (w.x - 1.0) * sc, ai = w.y * sc;
+    x[k].x += ar * s.x - ai * s.y;
+    x[k].y += ar * s.y + ai * s.x;
+    const double2 dk = d[k], ro = r_old[k];
+    r[k] = make_double2(ro.x - (w.x * dk.x - w.y * dk.y), ro.y - (w.x * dk.y + w.y * dk.x));
+}
+
+// undo the last refinement step of the flagged systems: x64 -= w ||r64_prev|| dx
 template <typename R, int F>
 __global__ void k_cg_rollback(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm,
-                              const int* __restrict__ flag, double2* __restrict__ x) {
+                              const double2* __restrict__ om, const int* __restrict__ flag, double2* __restrict__ x) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nq || !flag[k % F]) return;
     const double sc = sqrt(fmax(nrm[k % F].x, 0.0));
+    const double2 w = om[k % F];
     const double2 d = to_d(dx[k]);
-    x[k].x -= sc * d.x;
-    x[k].y -= sc * d.y;
+    x[k].x -= sc * (w.x * d.x - w.y * d.y);
+    x[k].y -= sc * (w.x * d.y + w.y * d.x);
 }
 
 template <typename R, int F>
@@ -1535,15 +1558,45 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     // (a step that made the residual worse is undone first)
     std::vector<double> best_out(F, HUGE_VAL), prev_rel(F, 1.0);
     std::vector<int> cyc_best(F, 0), gave_up(F, 0), rb(F, 0);
-    DBuf<double2> nrm_prev;
+    DBuf<double2> nrm_prev, r_old, dn, dd, omd;
     DBuf<int> rbd;
     EMG_HIP(c, nrm_prev.alloc(F));
     EMG_HIP(c, rbd.alloc(F));
+    EMG_HIP(c, r_old.alloc(nq));
+    EMG_HIP(c, dn.alloc(F));
+    EMG_HIP(c, dd.alloc(F));
+    EMG_HIP(c, omd.alloc(F));
+    auto dot64 = [&](const double2* a, const double2* b, double2* out) {  // a^H b per system
+        k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, b, partial.p);
+        k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
+    };
+    std::vector<double2> hdn(F), hdd(F), hom(F, make_double2(1.0, 0.0));
+    std::vector<int> stepped(F, 0);  // the system took an inner correction last cycle
+    EMG_HIP(c, hipMemcpyAsync(omd.p, hom.data(), sizeof(double2) * F, hipMemcpyHostToDevice, s));
     for (int cyc = 0; cyc < P.max_cycles; ++cyc) {
         if (cyc > 0) EMG_HIP(c, hipMemcpyAsync(nrm_prev.p, nrm.p, sizeof(double2) * F, hipMemcpyDeviceToDevice, s));
         k_spmv<double, F><<<grow, 256, 0, s>>>(op64, k264.p, gam64.p, x64, nullptr, t64.p);
         k_sub<<<gq, 256, 0, s>>>(nq, b64, t64.p, r64.p);
+        if (cyc > 0) {
+            // minimal-residual scaling of the last correction (t64 <- d)
+            k_sub<<<gq, 256, 0, s>>>(nq, r_old.p, r64.p, t64.p);
+            dot64(t64.p, r_old.p, dn.p);
+            dot64(t64.p, t64.p, dd.p);
+            EMG_HIP(c, hipMemcpyAsync(hdn.data(), dn.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipMemcpyAsync(hdd.data(), dd.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipStreamSynchronize(s));
+            for (int f = 0; f < F; ++f) {
+                const double den = hdd[f].x;
+                hom[f] = (stepped[f] && den > 0 && std::isfinite(den) && std::isfinite(hdn[f].x) &&
+                          std::isfinite(hdn[f].y))
+                             ? make_double2(hdn[f].x / den, hdn[f].y / den)
+                             : make_double2(1.0, 0.0);
+            }
+            EMG_HIP(c, hipMemcpyAsync(omd.p, hom.data(), sizeof(double2) * F, hipMemcpyHostToDevice, s));
+            k_mr_apply<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, omd.p, x64, r_old.p, t64.p, r64.p);
+        }
         dots64(r64.p, nrm.p);
+        EMG_HIP(c, hipMemcpyAsync(r_old.p, r64.p, sizeof(double2) * nq, hipMemcpyDeviceToDevice, s));
         EMG_HIP(c, hipMemcpyAsync(hr.data(), nrm.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
         EMG_HIP(c, hipStreamSynchronize(s));
         EMG_HIP(c, hipGetLastError());
@@ -1575,10 +1628,11 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         st.cycles = cyc;
         if (std::find(rb.begin(), rb.end(), 1) != rb.end()) {
             EMG_HIP(c, hipMemcpyAsync(rbd.p, rb.data(), sizeof(int) * F, hipMemcpyHostToDevice, s));
-            k_cg_rollback<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, rbd.p, x64);
+            k_cg_rollback<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, omd.p, rbd.p, x64);
             EMG_HIP(c, hipGetLastError());
         }
         if (all) break;
+        for (int f = 0; f < F; ++f) stepped[f] = ht[f] < 1.0;
         EMG_HIP(c, hipMemcpyAsync(tgt2.p, ht.data(), sizeof(double) * F, hipMemcpyHostToDevice, s));
         k_cast_scaled<R, F><<<gq, 256, 0, s>>>(nq, r64.p, nrm.p, r.p);
         k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 1, int(aux));
@@ -1616,8 +1670,11 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         // ones got a target they already meet)
         for (int f = 0; f < F; ++f) active[f] = ht[f] < 1.0;
         // a block runs until its slowest system stops: once the first system
-        // has converged at iteration k1, systems still running past
+        // has converged at iteration k1, complex64 systems still running past
         // max(4 k1, k1 + 400) are stopped and restarted by the refinement
+        // (complex64 inner solves can wander; complex128 ones near the end of
+        // the refinement have loose targets, and cutting them would stall
+        // the slower systems)
         int first_done = -1;
         while (it < P.restart) {
             if (use_graph) EMG_HIP(c, hipGraphLaunch(graph, s));
@@ -1632,7 +1689,8 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             if (first_done < 0)
                 for (int f = 0; f < F; ++f)
                     if (active[f] && hdone[f]) first_done = it;
-            const bool cut = first_done >= 0 && it > std::max(4 * first_done, first_done + 400);
+            const bool cut = sizeof(R) < sizeof(double) && first_done >= 0 &&
+                             it > std::max(4 * first_done, first_done + 400);
             for (int f = 0; f < F; ++f) {
                 if (cut && !hdone[f] && active[f]) {
                     hdone[f] = 1;
