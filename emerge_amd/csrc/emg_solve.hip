@@ -1915,8 +1915,9 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
             EMG_HIP(c, flag.alloc(1));
             EMG_HIP(c, hipMemsetAsync(flag.p, 0, sizeof(int), s));
             EMG_HIP(c, c->E64r.alloc(std::max<int64_t>(c->nnz_red, 1)));
-            k_real_part<<<grid_for(c->nnz_red, 256), 256, 0, s>>>(
-                c->nnz_red, reinterpret_cast<const double2*>(c->E64.p), c->E64r.p, flag.p);
+            if (c->nnz_red > 0)
+                k_real_part<<<grid_for(c->nnz_red, 256), 256, 0, s>>>(
+                    c->nnz_red, reinterpret_cast<const double2*>(c->E64.p), c->E64r.p, flag.p);
             EMG_HIP(c, hipGetLastError());
             int hf = 0;
             EMG_HIP(c, hipMemcpyAsync(&hf, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));

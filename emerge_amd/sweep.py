@@ -104,6 +104,7 @@ class FrequencySweep:
         self.eps_r = np.asarray(eps_r, dtype=np.complex128)
         self._aux = None          # host arrays of the auxiliary gradient space, built on first use
         self._aux_on = False
+        self._escalated = 0       # systems finished with complex128 inner solves in the last solve
         self.bcs = list(boundary_conditions)
         self.pec = [b for b in self.bcs if isinstance(b, bcs.PEC)]
         self.robin = [b for b in self.bcs if isinstance(b, bcs.RobinBC)]
