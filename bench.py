@@ -300,6 +300,22 @@ def main() -> int:
     print(f"[bench] mesh T={m.n_tets} V={m.n_nodes} N={f.n_field} ({t_mesh:.1f} s)", file=sys.stderr, flush=True)
 
     ctx = Context(0)
+    # mesh topology (E/mesh3d.py:386-498): host NumPy restatement vs the
+    # device build (checked bit-exact here as well)
+    from emerge_amd.mesh import Mesh3D
+    t0 = time.perf_counter()
+    mh = Mesh3D(m.nodes, m.tets)
+    t_topo_host = time.perf_counter() - t0
+    Mesh3D.on_device(m.nodes, m.tets, ctx)  # warm (code objects, allocations)
+    t0 = time.perf_counter()
+    md = Mesh3D.on_device(m.nodes, m.tets, ctx)
+    t_topo_dev = time.perf_counter() - t0
+    topo_exact = all(np.array_equal(getattr(mh, k), getattr(md, k))
+                     for k in ("edges", "tris", "tet_to_edge", "tet_to_tri", "tri_to_edge", "tri_to_tet",
+                               "edge_lengths"))
+    del mh, md
+    print(f"[bench] topology host {t_topo_host:.2f} s, device {t_topo_dev:.3f} s (bit-exact: {topo_exact})",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     dm = MeshOnDevice(ctx, m.nodes, m.tets, f.tet_to_field, f.n_field)
     ctx.synchronize()
@@ -363,6 +379,8 @@ def main() -> int:
                         "eps_r=U[1,10](1-j U[0,0.02]), mu_r=1; FP64 Nedelec-2 E and B into CSR",
             "tets": T, "dofs": f.n_field, "nnz": nnz,
             "pattern_build_s": round(t_pattern, 3),
+            "topology_host_s": round(t_topo_host, 3), "topology_device_s": round(t_topo_dev, 4),
+            "topology_bit_exact": bool(topo_exact),
         },
         "roofline": {
             "bound": "hbm",

@@ -20,6 +20,7 @@ SOURCES = [
     "emg_surface.hip",
     "emg_solve.hip",
     "emg_sparam.hip",
+    "emg_topology.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")

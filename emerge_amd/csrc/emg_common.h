@@ -131,6 +131,12 @@ struct emg_ctx {
     emg::DBuf<double> probe_wa;           // [n] weight
     std::vector<double> port_norm;        // sum w |c|^2 per port
 
+    // mesh topology built on the device (emg_topology_build)
+    int64_t topo_T = 0, topo_nE = 0, topo_nF = 0;
+    emg::DBuf<double> topo_xyz, topo_len;
+    emg::DBuf<uint64_t> topo_ekeys, topo_tkeys;
+    emg::DBuf<int32_t> topo_edges, topo_tris, topo_t2e, topo_t2t, topo_r2e, topo_r2t;
+
     // Krylov method of the solves (emg_solver_config)
     int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part
     int idr_s = 4;

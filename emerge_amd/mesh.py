@@ -116,6 +116,43 @@ class Mesh3D:
         d = self.nodes[:, self.edges[0]] - self.nodes[:, self.edges[1]]
         self.edge_lengths = np.sqrt(np.sum(d ** 2, axis=0))
 
+    @classmethod
+    def on_device(cls, nodes: np.ndarray, tets: np.ndarray, ctx) -> "Mesh3D":
+        """The same topology built on the GPU (``emg_topology_build``: sort-
+        unique keys and binary searches), bit-exact with ``_build``."""
+        import ctypes
+
+        from ._lib import ptr
+        obj = cls.__new__(cls)
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        tets = np.ascontiguousarray(tets, dtype=np.int64)
+        if nodes.ndim != 2 or nodes.shape[0] != 3 or tets.ndim != 2 or tets.shape[0] != 4:
+            raise ValueError("nodes must have shape (3, V) and tets (4, T)")
+        nv, T = nodes.shape[1], tets.shape[1]
+        obj.nodes, obj.tets = nodes, tets
+        t32 = np.ascontiguousarray(tets, dtype=np.int32)
+        nE, nF = ctypes.c_int64(), ctypes.c_int64()
+        ctx.call("emg_topology_build", nv, ptr(nodes), T, ptr(t32), ctypes.byref(nE), ctypes.byref(nF))
+        nE, nF = nE.value, nF.value
+        edges = np.empty((2, nE), dtype=np.int32)
+        tris = np.empty((3, nF), dtype=np.int32)
+        t2e = np.empty((6, T), dtype=np.int32)
+        t2t = np.empty((4, T), dtype=np.int32)
+        r2e = np.empty((3, nF), dtype=np.int32)
+        r2t = np.empty((2, nF), dtype=np.int32)
+        lengths = np.empty(nE, dtype=np.float64)
+        ctx.call("emg_topology_get", ptr(edges), ptr(tris), ptr(t2e), ptr(t2t), ptr(r2e), ptr(r2t), ptr(lengths))
+        obj.edges = edges.astype(np.int64)
+        obj.tris = tris.astype(np.int64)
+        obj.tet_to_edge = t2e.astype(np.int64)
+        obj.tet_to_tri = t2t.astype(np.int64)
+        obj.tri_to_edge = r2e.astype(np.int64)
+        obj.tri_to_tet = r2t.astype(np.int64)
+        obj.edge_lengths = lengths
+        obj._edge_keys = obj.edges[0] * nv + obj.edges[1]
+        obj._tri_keys = (obj.tris[0] * nv + obj.tris[1]) * nv + obj.tris[2]
+        return obj
+
     # ------------------------------------------------------------- properties
     @property
     def n_nodes(self) -> int:

@@ -228,6 +228,21 @@ int emg_solver_set_escalation(emg_ctx* ctx, int on);
  * reference counterpart (the reference solves in FP64). */
 int emg_solver_set_safe32(emg_ctx* ctx, int on);
 
+/* Mesh topology on the device (replaces Mesh3D._pre_update,
+ * E/mesh3d.py:386-498): unique edges / triangles of the tets (sorted vertex
+ * tuples in lexicographic order) and the maps tet_to_edge (6 x T, local
+ * pairs (0,1),(0,2),(0,3),(1,2),(3,1),(2,3)), tet_to_tri (4 x T, local
+ * triples (0,1,2),(0,2,3),(0,3,1),(1,2,3)), tri_to_edge (3 x nF),
+ * tri_to_tet (2 x nF, -1234 for a missing second tet) and edge lengths,
+ * bit-exact with the reference.  xyz_soa (3 x V), tets (4 x T) int32
+ * row-major; V < 2^21. */
+int emg_topology_build(emg_ctx* ctx, int64_t V, const double* xyz_soa, int64_t T, const int32_t* tets,
+                       int64_t* n_edges, int64_t* n_tris);
+/* Copies the built topology out (row-major int32 arrays as above; any
+ * pointer may be null to skip that array). */
+int emg_topology_get(emg_ctx* ctx, int32_t* edges, int32_t* tris, int32_t* tet_to_edge, int32_t* tet_to_tri,
+                     int32_t* tri_to_edge, int32_t* tri_to_tet, double* edge_lengths);
+
 /* Auxiliary gradient space of the COCG preconditioner (method 2 of
  * emg_sweep_solve): the inner solves apply z = D^-1 r + G D_g^-1 G^T r with
  * D the diagonal of the operator, G (n_red x n_nodal, real CSR g_*, its
