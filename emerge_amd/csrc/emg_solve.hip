@@ -1054,8 +1054,8 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
 // A lane covers V consecutive systems of the batch (V/2 16-byte gathers per
 // matrix entry) and every GR-th entry of its row, two entries per step so
 // that their gathers overlap.  A row gets LPR = FG * GR lanes with GR = 16
-// entries in flight (rows hold ~45 entries), so small batches put several
-// rows in one wavefront instead of leaving lanes idle.  The butterfly
+// entries in flight (8 at F >= 32; rows hold ~40 entries), so small batches
+// put several rows in one wavefront instead of leaving lanes idle.  The butterfly
 // reduction stays inside the row's lanes and leaves the row sum in all GR
 // lanes of its systems, so the row end (surface terms, q store, p^T q) is
 // spread one system per lane, and those lanes load their p before the
@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     constexpr int V = F >= VM ? VM : F;         // systems per lane
     constexpr int FG = F / V;                   // lanes per entry
     constexpr int GW = 64 / FG;                 // entry lanes per wavefront
-    constexpr int GR = GW >= 16 ? 16 : GW;      // entries in flight per row (GR >= V)
+    constexpr int GR = GW >= 16 ? (F >= 32 ? 8 : 16) : GW;  // entries in flight per row (GR >= V)
     constexpr int LPR = FG * GR;                // lanes per row
     constexpr int RPW = 64 / LPR;               // rows per wavefront
     constexpr int W = kCgThreads / 64;
