@@ -474,15 +474,17 @@ class FrequencySweep:
         self.tuning = dict(cache=dict(cache), seconds_per_system_iteration=timings)
         return best_f
 
-    def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock=16, restart: int = 100,
-              max_cycles: int = 200, keep_fields: bool = True, method: str = "gmres", idr_s: int = 4,
-              inner_rel32: float = 1e-5, precond: str = "aux", fallback: bool = True,
+    def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock=32, restart: int | None = None,
+              max_cycles: int = 200, keep_fields: bool = True, method: str = "cocg", idr_s: int = 4,
+              inner_rel32: float | None = None, precond: str = "aux", fallback: bool = True,
               safe32="auto") -> SweepResult:
         """Solve every (frequency, active port) system.  method 'gmres' =
         GMRES(restart); 'idr' = IDR(idr_s) with a budget of `restart`
         matvecs per FP64 refinement cycle; 'cocg' = COCG on the symmetric
         part inside FP64 refinement, at most `restart` iterations per cycle,
-        each cycle reducing the residual by `inner_rel32`.  precond (COCG
+        each cycle reducing the residual by `inner_rel32` (defaults per
+        method: restart 100 / 30000 / 20000, inner_rel32 1e-5 / 1e-4 / 1e-2
+        for gmres / idr / cocg).  precond (COCG
         only): 'aux' = Jacobi plus the auxiliary gradient-space correction
         (``aux_space``), 'jacobi' = Jacobi alone.  fallback (mixed only):
         systems that miss tol are solved again in FP64
@@ -492,8 +494,14 @@ class FrequencySweep:
         SpMV and the gradient-space correction in FP64; 'auto' turns it on
         when ``needs_safe32`` says the mesh is fine against the wavelength."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+        if method not in ("gmres", "idr", "cocg"):
+            raise ValueError("method must be 'gmres', 'idr' or 'cocg'")
         if precond not in ("aux", "jacobi"):
             raise ValueError("precond must be 'aux' or 'jacobi'")
+        if restart is None:
+            restart = {"gmres": 100, "idr": 30000, "cocg": 20000}[method]
+        if inner_rel32 is None:
+            inner_rel32 = {"gmres": 1e-5, "idr": 1e-4, "cocg": 1e-2}[method]
         if safe32 == "auto":
             safe32 = self.needs_safe32(freqs)
         self.ctx.call("emg_solver_set_safe32", int(bool(safe32)))
