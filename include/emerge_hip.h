@@ -165,9 +165,12 @@ int emg_forcing_get(emg_ctx* ctx, double* out);
  * (gamma complex [nbatch][n_bc], coef complex [nbatch][n_vectors]).
  * Replaces the per-frequency assemble + SolveRoutine.solve of run_sweep
  * (MW/microwave_3d.py:773-794; E/solver.py:1241-1326).  precision 0 = FP64
- * GMRES, 1 = complex64 GMRES inside FP64 iterative refinement; fblock = max
- * systems per batch (1..32, matrix read once per batch).  x_out complex
- * [nbatch][n_red]; relres_out = final FP64 ||b - A x|| / ||b||. */
+ * inner solves, 1 = complex64 inner solves inside FP64 iterative refinement
+ * (with COCG, systems whose complex64 inner solves stall are finished with
+ * complex128 inner solves from their iterate: emg_sweep_escalated); the
+ * method is emg_solver_config's; fblock = max systems per batch (1..32,
+ * matrix read once per batch).  x_out complex [nbatch][n_red]; relres_out =
+ * final FP64 ||b - A x|| / ||b||. */
 int emg_sweep_solve(emg_ctx* ctx, int64_t nbatch, const double* k2, const double* gamma,
                     const double* rhs_coef, int precision, int fblock, double tol, int restart,
                     int max_cycles, double* x_out, double* relres_out, int64_t* iters_out,
@@ -205,8 +208,12 @@ int emg_port_norms_get(emg_ctx* ctx, double* norms);
 
 /* Krylov method for emg_sweep_solve / emg_csr_solve: 0 = GMRES(restart)
  * with CGS2 (default), 1 = IDR(idr_s) with bi-orthogonalisation (then
- * `restart` is the matvec budget of one refinement cycle); inner_rel32 = the
- * residual reduction an FP32 inner solve aims for before the FP64 residual is
+ * `restart` is the matvec budget of one refinement cycle), 2 = COCG on the
+ * complex-symmetric part (A + A^T)/2 inside FP64 refinement on A (then
+ * `restart` caps the inner iterations of one cycle; Jacobi preconditioning,
+ * plus the gradient-space correction of emg_aux_set when set; each
+ * correction is scaled to minimise the FP64 residual); inner_rel32 = the
+ * residual reduction an inner solve aims for before the FP64 residual is
  * recomputed.  The reference's iterative solvers (scipy gmres/bicgstab/
  * gcrotmk, E/solver.py:458-527) have no counterpart knob; this selects the
  * device algorithm. */
