@@ -1351,6 +1351,59 @@ __global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restr
     if (j < op.aux_n && g == 0) t[j * F + f] = cmul(dginv[j * F + f], cmk<A>(ax, ay));
 }
 
+// The same for batches of F >= 8 systems: a lane gathers V consecutive
+// systems of an entry (64 bytes, as the SpMV), F / V lanes per entry and
+// 64 V / F entries in flight per nodal row, one row per wavefront; the
+// butterfly leaves each row sum in the lanes of its systems and lane g of a
+// system group writes system f0 + g.
+template <typename R, int F, typename A = R>
+__global__ __launch_bounds__(256) void k_aux_gt_v(Op<R> op, const cplx<R>* __restrict__ r,
+                                                  const cplx<A>* __restrict__ dginv, cplx<A>* __restrict__ t) {
+    constexpr int VM = sizeof(R) == 4 ? 8 : 4;
+    constexpr int V = F >= VM ? VM : F;
+    constexpr int FG = F / V;
+    constexpr int G = 64 / FG;
+    const int lane = threadIdx.x & 63;
+    const int fg = lane % FG, g = lane / FG, f0 = fg * V;
+    const int64_t j = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    A ax[V], ay[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) ax[v] = ay[v] = A(0);
+    if (j < op.aux_n) {
+        const int e1 = op.tptr[j + 1];
+        for (int e = op.tptr[j] + g; e < e1; e += G) {
+            A w;
+            if constexpr (sizeof(A) > sizeof(R)) w = op.tval_d[e];
+            else w = op.tval[e];
+            const cplx<R>* rc = r + int64_t(op.tidx[e]) * F + f0;
+            cplx<R> rv[V];
+#pragma unroll
+            for (int v = 0; v < V; v += 2) load_pair(rc + v, rv[v], rv[v + 1]);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                ax[v] += w * A(rv[v].x);
+                ay[v] += w * A(rv[v].y);
+            }
+        }
+    }
+    A sx = A(0), sy = A(0);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        A tx = ax[v], ty = ay[v];
+#pragma unroll
+        for (int off = 32; off >= FG; off >>= 1) {
+            tx += __shfl_xor(tx, off);
+            ty += __shfl_xor(ty, off);
+        }
+        sx = (v == g) ? tx : sx;
+        sy = (v == g) ? ty : sy;
+    }
+    if (j < op.aux_n && g < V) {
+        const int64_t o = j * F + f0 + g;
+        t[o] = cmul(dginv[o], cmk<A>(sx, sy));
+    }
+}
+
 // z = dinv o r + G t and per-workgroup partial sums of r^T z (same grid,
 // stride and partial layout as k_cg_update, which leaves the ||r||^2 half)
 template <typename R, int F, typename A = R>
@@ -1500,6 +1553,8 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     // k_aux_gt: 64 / LPR nodal rows per wavefront, 4 wavefronts per workgroup
     const int rpw_gt = F >= 16 ? 1 : 4;
     const unsigned ggt = aux ? grid_for((op.aux_n + rpw_gt - 1) / rpw_gt, 4) : 1;
+    // k_aux_gt_v: one nodal row per wavefront, 4 wavefronts per workgroup
+    const unsigned ggv = aux ? grid_for(op.aux_n, 4) : 1;
     if (aux && !safe) k_aux_dg<R, F><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv.p);
     if (aux && safe) k_aux_dg<R, F, double><<<gaux, 256, 0, s>>>(op, k264.p, gam64.p, dginv64.p);
     EMG_HIP(c, hipGetLastError());
@@ -1507,10 +1562,16 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     auto precondition = [&]() {
         if (!aux) return;
         if (safe) {
-            k_aux_gt<R, F, double><<<ggt, 256, 0, s>>>(op, r.p, dginv64.p, tg64.p);
+            if constexpr (F >= 8)
+                k_aux_gt_v<R, F, double><<<ggv, 256, 0, s>>>(op, r.p, dginv64.p, tg64.p);
+            else
+                k_aux_gt<R, F, double><<<ggt, 256, 0, s>>>(op, r.p, dginv64.p, tg64.p);
             k_aux_z<R, F, double><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg64.p, z.p, cs);
         } else {
-            k_aux_gt<R, F><<<ggt, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
+            if constexpr (F >= 8)
+                k_aux_gt_v<R, F><<<ggv, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
+            else
+                k_aux_gt<R, F><<<ggt, 256, 0, s>>>(op, r.p, dginv.p, tg.p);
             k_aux_z<R, F><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg.p, z.p, cs);
         }
     };
