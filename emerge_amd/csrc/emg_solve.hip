@@ -1066,7 +1066,8 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
                                                         const cplx<R>* __restrict__ gam,
                                                         const cplx<R>* __restrict__ p, cplx<R>* __restrict__ q,
                                                         CgState st) {
-    constexpr int VM = sizeof(R) == 4 ? 8 : 4;  // 8 complex64 or 4 complex128: 64 bytes per lane
+    // 8 complex64 or 4 complex128 per lane (64 bytes); 4 with FP64 accumulators
+    constexpr int VM = sizeof(A) == 4 ? 8 : 4;
     constexpr int V = F >= VM ? VM : F;         // systems per lane
     constexpr int FG = F / V;                   // lanes per entry
     constexpr int GW = 64 / FG;                 // entry lanes per wavefront
@@ -1359,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_aux_gt(Op<R> op, const cplx<R>* __restr
 template <typename R, int F, typename A = R>
 __global__ __launch_bounds__(256) void k_aux_gt_v(Op<R> op, const cplx<R>* __restrict__ r,
                                                   const cplx<A>* __restrict__ dginv, cplx<A>* __restrict__ t) {
-    constexpr int VM = sizeof(R) == 4 ? 8 : 4;
+    constexpr int VM = sizeof(A) == 4 ? 8 : 4;  // 4 systems per lane with FP64 accumulators
     constexpr int V = F >= VM ? VM : F;
     constexpr int FG = F / V;
     constexpr int G = 64 / FG;

@@ -331,12 +331,14 @@ def test_autotuned_block_width(sweep):
     print(f"autotuned fblock={f} timings={sweep.tuning}")
 
 
-@pytest.mark.parametrize("safe32", [True, False])
-def test_mixed_safe32_modes_match_superlu(sweep, safe32):
+@pytest.mark.parametrize("safe32,nf", [(True, 2), (False, 2), (True, 4)])
+def test_mixed_safe32_modes_match_superlu(sweep, safe32, nf):
     """Complex64 COCG with the FP64 real stiffness and FP64 accumulation
-    (safe32) and without it: both converge to the oracle's S."""
-    freqs = np.array([8e9, 12e9])
-    res = sweep.solve(freqs, precision="mixed", tol=1e-8, fblock=4, restart=20000, max_cycles=30,
+    (safe32) and without it: both converge to the oracle's S (nf = 4: a
+    block of 8 systems, the wide-batch kernels of the FP64-accumulating
+    path)."""
+    freqs = np.linspace(8e9, 12e9, nf)
+    res = sweep.solve(freqs, precision="mixed", tol=1e-8, fblock=2 * nf, restart=20000, max_cycles=30,
                       method="cocg", inner_rel32=1e-2, safe32=safe32)
     assert np.all(res.relres <= 1e-8), res.relres
     for i, f in enumerate(freqs):
