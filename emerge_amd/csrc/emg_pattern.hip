@@ -337,6 +337,7 @@ extern "C" int emg_pattern_build(emg_ctx* c, int64_t* nnz_out) {
     c->max_rank = max_rank;
     c->have_pattern = true;
     c->have_values = false;
+    c->have_system = false;  // the reduced map indexes the old pattern
     if (nnz_out) *nnz_out = nnz;
     return EMG_OK;
 }

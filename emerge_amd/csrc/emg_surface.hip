@@ -427,6 +427,27 @@ extern "C" int emg_system_reduce(emg_ctx* c, int64_t n_keep, const int32_t* keep
     return EMG_OK;
 }
 
+extern "C" int emg_system_refresh(emg_ctx* c) {
+    EMG_CHECK_CTX(c);
+    if (!c->have_system) return fail(c, EMG_ERR_STATE, "emg_system_refresh: no reduced system (emg_system_reduce)");
+    if (!c->have_values) return fail(c, EMG_ERR_STATE, "emg_system_refresh: assemble first");
+    hipStream_t st = c->stream;
+    const int64_t tot = c->nnz_red;
+    if (tot > 0) {
+        k_gather_vals<<<grid_for(tot, 256), 256, 0, st>>>(tot, c->red_map.p, reinterpret_cast<const double2*>(c->Ev.p),
+                                                          reinterpret_cast<double2*>(c->E64.p),
+                                                          reinterpret_cast<float2*>(c->E32.p));
+        k_gather_vals<<<grid_for(tot, 256), 256, 0, st>>>(tot, c->red_map.p, reinterpret_cast<const double2*>(c->Bv.p),
+                                                          reinterpret_cast<double2*>(c->B64.p),
+                                                          reinterpret_cast<float2*>(c->B32.p));
+        EMG_HIP(c, hipGetLastError());
+    }
+    EMG_HIP(c, hipStreamSynchronize(st));
+    c->have_bsym = false;  // derived from the old mass values
+    c->er64_state = 0;     // E may have turned complex (or real)
+    return EMG_OK;
+}
+
 extern "C" int emg_system_pattern_get(emg_ctx* c, int32_t* indptr, int32_t* indices, int64_t* nnz) {
     EMG_CHECK_CTX(c);
     if (!c->have_system) return fail(c, EMG_ERR_STATE, "emg_system_pattern_get: no reduced system");

@@ -92,20 +92,25 @@ def _tet_gradients(nodes: np.ndarray, tets: np.ndarray):
 
 
 def nodal_volume_diagonal(mesh, eps_r: np.ndarray) -> np.ndarray:
-    """int eps |grad phi_j|^2 dV for every nodal function (V + nE,), with
-    eps the tet's (scalar) permittivity: vertex hats sum eps V |grad lam|^2,
-    edge bubbles eps V/10 (|g_x|^2 + |g_y|^2 + g_x . g_y)."""
+    """int grad phi_j . (eps grad phi_j) dV for every nodal function
+    (V + nE,), with eps the tet's permittivity, a scalar (T,) or a 3x3
+    tensor (3, 3, T): vertex hats sum V g_a.eps g_a, edge bubbles
+    V/10 (g_x.eps g_x + g_y.eps g_y) + V/20 (g_x.eps g_y + g_y.eps g_x)
+    (int lam_a^2 = V/10, int lam_x lam_y = V/20)."""
     V, nE = mesh.n_nodes, mesh.n_edges
     g, vol = _tet_gradients(mesh.nodes, mesh.tets)
-    w = eps_r * vol                                      # (T,) complex
+    eps = np.asarray(eps_r)
     out = np.zeros(V + nE, dtype=np.complex128)
-    gg = np.einsum("tic,tjc->tij", g, g)                 # (T, 4, 4)
+    if eps.ndim == 3:
+        gg = np.einsum("tia,abt,tjb->tij", g, eps, g) * vol[:, None, None]   # (T, 4, 4) g_i.eps g_j V
+    else:
+        gg = np.einsum("tic,tjc->tij", g, g) * (eps * vol)[:, None, None]
     for i in range(4):
-        np.add.at(out, mesh.tets[i], w * gg[:, i, i])
+        np.add.at(out, mesh.tets[i], gg[:, i, i])
     pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
     for i, j in pairs:
         eid = mesh.edge_ids(mesh.tets[i], mesh.tets[j])
-        val = w / 10.0 * (gg[:, i, i] + gg[:, j, j] + gg[:, i, j])
+        val = (gg[:, i, i] + gg[:, j, j]) / 10.0 + (gg[:, i, j] + gg[:, j, i]) / 20.0
         np.add.at(out, V + eid, val)
     return out
 

@@ -26,6 +26,17 @@ from ._lib import Context, ptr
 from .assembler import MeshOnDevice
 
 C0 = 299792458
+EPS0 = 8.854187818814e-12
+
+
+def lossy_permittivity(eps_r, tand, cond, freq: float) -> np.ndarray:
+    """Complex relative permittivity with loss tangent and conductivity (S/m)
+    at `freq`: ``eps_r (1 - j tand) - j cond / (w eps0)``, w = 2 pi freq
+    (ASM/assembler.py:218, 240).  Arrays broadcast (scalars (T,) or tensors
+    (3, 3, T))."""
+    w0 = 2 * np.pi * freq
+    return np.asarray(eps_r) * (1 - 1j * np.asarray(tand)) - 1j * np.asarray(cond) / (w0 * EPS0)
+
 
 # gaus_quad_tri(4) (E/mth/optimized.py:31-32, 87-118): weights and
 # barycentric coordinates in generation order
@@ -91,18 +102,38 @@ def _morton_order(pos: np.ndarray, ids: np.ndarray, h: float) -> np.ndarray:
 class FrequencySweep:
     """Device-resident E, B, surface terms and forcing of one mesh."""
 
-    def __init__(self, mesh, basis, eps_r: np.ndarray, inv_mu_r: np.ndarray, boundary_conditions: list,
-                 device: int = 0, ctx: Context | None = None, dof_order: str = "spatial"):
+    def __init__(self, mesh, basis, eps_r, inv_mu_r, boundary_conditions: list,
+                 device: int = 0, ctx: Context | None = None, dof_order: str = "spatial", mu_r=None):
+        """Materials per tet: eps_r a complex scalar (T,) or tensor (3, 3, T);
+        inv_mu_r complex scalars 1/mu_r (T,), or mu_r a tensor (3, 3, T) (the
+        reference's ``ur``, inverted on the device) in its place.  Any of them
+        may be a callable of the frequency in Hz returning such an array:
+        the sweep is then frequency-dependent and every frequency is
+        re-assembled before its systems are solved, as the reference does
+        for frequency-dependent or conductive materials
+        (ASM/assembler.py:222-252; conductivity as ``eps_r - 1j*cond/(w eps0)``,
+        ``lossy_permittivity``)."""
         if dof_order not in ("spatial", "natural"):
             raise ValueError("dof_order must be 'spatial' or 'natural'")
+        if (inv_mu_r is None) == (mu_r is None):
+            raise ValueError("give exactly one of inv_mu_r (scalars) and mu_r (tensors)")
         self.dof_order = dof_order
         self.mesh = mesh
         self.basis = basis
         self.ctx = ctx if ctx is not None else Context(device)
         self.dm = MeshOnDevice(self.ctx, mesh.nodes, mesh.tets, basis.tet_to_field, basis.n_field)
-        self.dm.assemble(eps_r, inv_mu_r)
-        self.eps_r = np.asarray(eps_r, dtype=np.complex128)
+        self._materials = (eps_r, inv_mu_r, mu_r)
+        self.frequency_dependent = any(callable(v) for v in self._materials)
+        self._mat_freq = None     # frequency whose materials are assembled (frequency-dependent sweeps)
+        if self.frequency_dependent:
+            # vacuum placeholder for the reduction below; every solve assembles
+            # the materials of its own frequencies first
+            one = np.ones(mesh.n_tets, dtype=np.complex128)
+            self._assemble_materials(one, one, None)
+        else:
+            self._assemble_materials(eps_r, inv_mu_r, mu_r)
         self._aux = None          # host arrays of the auxiliary gradient space, built on first use
+        self._dvol = None         # its volume diagonal for the assembled materials
         self._aux_on = False
         self._escalated = 0       # systems finished with complex128 inner solves in the last solve
         self.bcs = list(boundary_conditions)
@@ -110,6 +141,50 @@ class FrequencySweep:
         self.robin = [b for b in self.bcs if isinstance(b, bcs.RobinBC)]
         self.ports = sorted([b for b in self.robin if b.is_port], key=lambda p: p.port_number)
         self._setup()
+
+    # -------------------------------------------------------------- materials
+    def _assemble_materials(self, eps_r, inv_mu_r, mu_r) -> None:
+        """Upload one material set and assemble E and B: the isotropic kernel
+        for scalar eps_r and inv_mu_r, the tensor kernel otherwise (scalars
+        become multiples of the identity, 1/inv_mu_r for mu_r)."""
+        T = self.mesh.n_tets
+        e = np.asarray(eps_r, dtype=np.complex128)
+        if e.shape not in ((T,), (3, 3, T)):
+            raise ValueError("eps_r must have shape (n_tets,) or (3, 3, n_tets)")
+        eye = np.eye(3)[:, :, None]
+        if mu_r is None and e.ndim == 1:
+            self.dm.assemble(e, np.asarray(inv_mu_r, dtype=np.complex128))
+        else:
+            et = e if e.ndim == 3 else e[None, None, :] * eye
+            if mu_r is None:
+                inu = np.asarray(inv_mu_r, dtype=np.complex128)
+                if inu.shape != (T,):
+                    raise ValueError("inv_mu_r must have shape (n_tets,); give mu_r for tensors")
+                mt = (1.0 / inu)[None, None, :] * eye
+            else:
+                mt = np.asarray(mu_r, dtype=np.complex128)
+                mt = mt if mt.ndim == 3 else mt[None, None, :] * eye
+            self.ctx.material_set_tensor(et, mt)
+            self.ctx.assemble()
+        self.eps_r = e
+        self._dvol = None
+
+    def materials_at(self, freq: float):
+        """(eps_r, inv_mu_r, mu_r) at `freq` (callables evaluated)."""
+        return tuple(v(freq) if callable(v) else v for v in self._materials)
+
+    def _load_frequency(self, freq: float) -> None:
+        """Assemble the materials of `freq` into the device system (frequency-
+        dependent sweeps): new E and B values on the same pattern, gathered
+        into the reduced system; surface terms, forcing and ports stay."""
+        if self._mat_freq == freq:
+            return
+        self._assemble_materials(*self.materials_at(freq))
+        self.ctx.call("emg_system_refresh")
+        self._mat_freq = freq
+        if self._aux_on:          # the gradient-space diagonal follows eps
+            self._aux_on = False
+            self._set_aux(True)
 
     # ------------------------------------------------------------------ setup
     def pec_dofs(self) -> np.ndarray:
@@ -278,8 +353,9 @@ class FrequencySweep:
         exact volume integral int eps |grad phi|^2 and, per Robin term,
         int_S |grad_t phi|^2 (second-order ABC corrections contribute none).
         Returns (G csr, dvol, dsurf)."""
+        m = self.mesh
         if self._aux is None:
-            m, f = self.mesh, self.basis
+            f = self.basis
             G = gradient.discrete_gradient(m, f)
             V = m.n_nodes
             drop = np.zeros(G.shape[1], dtype=bool)
@@ -292,13 +368,15 @@ class FrequencySweep:
             cols = cols[_morton_order(pos, cols, float(np.median(m.edge_lengths)))]
             Gr = G[self.solve_ids][:, cols].tocsr()
             Gr.sort_indices()
-            dvol = gradient.nodal_volume_diagonal(m, self.eps_r)[cols]
             dsurf = np.zeros((len(self.surface), cols.shape[0]))
             for i, (kind, b) in enumerate(self.surface):
                 if kind == "robin":
                     dsurf[i] = gradient.nodal_surface_diagonal(m, b.tri_ids)[cols]
-            self._aux = (Gr, dvol, dsurf)
-        return self._aux
+            self._aux = (Gr, cols, dsurf)
+        Gr, cols, dsurf = self._aux
+        if self._dvol is None:
+            self._dvol = gradient.nodal_volume_diagonal(m, self.eps_r)[cols]
+        return Gr, self._dvol, dsurf
 
     def _set_aux(self, on: bool) -> None:
         if on == self._aux_on:
@@ -505,31 +583,33 @@ class FrequencySweep:
         if safe32 == "auto":
             safe32 = self.needs_safe32(freqs)
         self.ctx.call("emg_solver_set_safe32", int(bool(safe32)))
-        if fblock == "auto":
+        npt = len(self.ports)
+        if self.frequency_dependent:
+            # one frequency per device batch: each has its own E and B
+            fblock = max(1, npt)
+        elif fblock == "auto":
             fblock = self.autotune_fblock(freqs, precision=precision, method=method, precond=precond,
                                           idr_s=idr_s, inner_rel32=inner_rel32)
-        self._set_aux(method == "cocg" and precond == "aux")
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s),
                       float(inner_rel32))
         k2, gam, coef = self.batch_coefficients(freqs)
-        npt = len(self.ports)
         self._escalated = 0
         run = dict(fblock=fblock, tol=tol, restart=restart, max_cycles=max_cycles, keep_fields=keep_fields)
-        x, relres, iters, ov = self._run_rows(k2, gam, coef, {"fp64": 0, "mixed": 1}[precision], **run)
-        fell_back = np.zeros(k2.shape[0], dtype=bool)
-        if precision == "mixed" and fallback:
-            # systems the complex64 inner solves could not bring to tol (the
-            # refinement gave them up) are solved again in FP64
-            bad = np.nonzero(~(relres <= tol))[0]
-            if bad.size:
-                xb, rb, ib, ob = self._run_rows(k2[bad], gam[bad] if gam.size else gam, coef[bad], 0, **run)
-                relres[bad] = rb
-                iters[bad] += ib
-                if x is not None:
-                    x[bad] = xb
-                if ov is not None:
-                    ov[bad] = ob
-                fell_back[bad] = True
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        aux = method == "cocg" and precond == "aux"
+        if not self.frequency_dependent:
+            self._set_aux(aux)
+            x, relres, iters, ov, fell_back = self._solve_rows(k2, gam, coef, prec, fallback, run)
+        else:
+            parts = []
+            for i, fq in enumerate(freqs):
+                rows = np.arange(i * npt, (i + 1) * npt)
+                self._load_frequency(float(fq))
+                self._set_aux(aux)
+                parts.append(self._solve_rows(k2[rows], gam[rows] if gam.size else gam, coef[rows], prec,
+                                              fallback, run))
+            x, relres, iters, ov, fell_back = (
+                None if parts[0][j] is None else np.concatenate([q[j] for q in parts]) for j in range(5))
         fields = {}
         if x is not None:
             for i in range(freqs.shape[0]):
@@ -540,6 +620,24 @@ class FrequencySweep:
         S = self.s_matrix(ov, freqs) if ov is not None else None
         return SweepResult(freqs, self.ports, fields, relres.reshape(-1, npt), iters.reshape(-1, npt), precision,
                            S=S, fp64_fallback=fell_back.reshape(-1, npt), fp64_escalated=self._escalated)
+
+    def _solve_rows(self, k2, gam, coef, prec, fallback, run):
+        """`_run_rows`, then (mixed precision, fallback on) the rows whose
+        complex64 inner solves the refinement gave up solved again in FP64."""
+        x, relres, iters, ov = self._run_rows(k2, gam, coef, prec, **run)
+        fell_back = np.zeros(k2.shape[0], dtype=bool)
+        if prec == 1 and fallback:
+            bad = np.nonzero(~(relres <= run["tol"]))[0]
+            if bad.size:
+                xb, rb, ib, ob = self._run_rows(k2[bad], gam[bad] if gam.size else gam, coef[bad], 0, **run)
+                relres[bad] = rb
+                iters[bad] += ib
+                if x is not None:
+                    x[bad] = xb
+                if ov is not None:
+                    ov[bad] = ob
+                fell_back[bad] = True
+        return x, relres, iters, ov, fell_back
 
     def _run_rows(self, k2, gam, coef, prec, fblock, tol, restart, max_cycles, keep_fields):
         """One emg_sweep_solve over the given (frequency, port) rows."""

@@ -117,6 +117,14 @@ int emg_values_get(emg_ctx* ctx, double* E_vals, double* B_vals);
  * device and gathers the assembled E and B values into it (FP64 and FP32
  * copies). */
 int emg_system_reduce(emg_ctx* ctx, int64_t n_keep, const int32_t* keep_ids);
+/* Gather the values of a new emg_assemble (new materials on the same mesh
+ * and pattern) into the existing reduced system, keeping its pattern, the
+ * surface terms, forcing vectors, port probes and the auxiliary gradient
+ * space: the per-frequency re-assembly of frequency-dependent materials
+ * (ASM/assembler.py:222-252, where conductivity enters eps as
+ * -j cond / (omega eps0), :239-241).  A new mesh or pattern invalidates the
+ * reduced system (EMG_ERR_STATE here until emg_system_reduce runs again). */
+int emg_system_refresh(emg_ctx* ctx);
 /* Copy the reduced pattern (indptr n_keep + 1, indices nnz); either may be
  * NULL.  nnz receives the reduced nonzero count. */
 int emg_system_pattern_get(emg_ctx* ctx, int32_t* indptr, int32_t* indices, int64_t* nnz);

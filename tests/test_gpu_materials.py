@@ -1,0 +1,136 @@
+"""Tensor and frequency-dependent materials in the sweep, against the oracle.
+
+WR-90 guide (11 x 5 x 18 Kuhn cells) with a block of material between
+z = 15 and 25 mm.  Oracle side: E and B from the oracle's assembly with the
+same (3, 3, T) tensors (re-assembled per frequency when the material depends
+on it, ASM/assembler.py:222-252), then SuperLU with the reference's options
+and the oracle's S extraction.
+
+Bar: FP64 relative residual <= 1e-8 on the oracle's matrix and |dS| <= 1e-6.
+"""
+import functools
+
+import numpy as np
+import pytest
+
+from emerge_amd._lib import Context, EmgError
+from emerge_amd.sweep import FrequencySweep, lossy_permittivity
+from oracle import curlcurl as cc
+from oracle import fem
+from tests import wg_case
+
+pytestmark = pytest.mark.gpu
+
+FREQS = np.array([8.5e9, 10e9, 11.5e9])
+
+
+def _block(m):
+    zc = m.nodes[2, m.tets].mean(axis=0)
+    return (zc > 15e-3) & (zc < 25e-3)
+
+
+def _tensor_materials(T, blk):
+    er = np.zeros((3, 3, T), dtype=np.complex128)
+    ur = np.zeros((3, 3, T), dtype=np.complex128)
+    for k in range(3):
+        er[k, k] = 1.0
+        ur[k, k] = 1.0
+    er[0, 0, blk] = 2.2 * (1 - 0.01j)
+    er[1, 1, blk] = 3.0 * (1 - 0.02j)
+    er[2, 2, blk] = 2.6
+    er[0, 1, blk] = er[1, 0, blk] = 0.3
+    ur[1, 1, blk] = 1.2
+    ur[2, 2, blk] = 1.1 - 0.01j
+    return er, ur
+
+
+def _lossy_eps(T, blk, f):
+    eps0 = np.where(blk, 2.5, 1.0).astype(np.complex128)
+    return lossy_permittivity(eps0, np.where(blk, 0.005, 0.0), np.where(blk, 0.2, 0.0), f)
+
+
+@functools.lru_cache(maxsize=2)
+def _oracle_EB(kind: str, freq: float):
+    c = wg_case.case()
+    m = c["mesh"]
+    T = m.n_tets
+    if kind == "tensor":
+        er, ur = _tensor_materials(T, _block(m))
+    else:
+        er, ur = cc.isotropic(_lossy_eps(T, _block(m), freq)), cc.isotropic(np.ones(T, dtype=np.complex128))
+    return cc.tet_mass_stiffness_matrices(m, c["basis"].tet_to_field, er, ur)
+
+
+@functools.lru_cache(maxsize=8)
+def _oracle(kind: str, freq: float):
+    """(K, b, port vectors, ids, S) of the oracle at `freq`; the tensor
+    material does not depend on the frequency (one assembly)."""
+    c = wg_case.case()
+    E, B = _oracle_EB(kind, 0.0 if kind == "tensor" else freq)
+    K, b, pv, ids = fem.assemble_freq_matrix(c["mesh"], c["maps"], E, B, c["pec"], c["o_ports"], freq)
+    fields = fem.solve_all_ports(K, b, pv, ids)
+    S = fem.s_parameters(c["mesh"], c["basis"].tet_to_field, fields, c["o_ports"], 2 * np.pi * freq / fem.C0)
+    return K.tocsr(), b, pv, ids, S
+
+
+def _check(sw, res, kind):
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, f in enumerate(FREQS):
+        K, b, pv, ids, S = _oracle(kind, float(f))
+        for p in sw.ports:
+            x = res.fields[(i, p.port_number)]
+            bb = b + pv[p.port_number]
+            rr = np.linalg.norm((bb - K @ x)[ids]) / np.linalg.norm(bb[ids])
+            assert rr <= 1e-8, (f, p.port_number, rr)
+        dS = np.abs(res.S[i] - S).max()
+        print(f"f={f:.3e} dS={dS:.2e} iters={res.iters[i].tolist()}", flush=True)
+        assert dS <= 1e-6, (f, dS)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_anisotropic_tensor_block_matches_superlu(precision):
+    print(f"tensor block precision={precision}", flush=True)
+    c = wg_case.case()
+    m = c["mesh"]
+    er, ur = _tensor_materials(m.n_tets, _block(m))
+    sw = FrequencySweep(m, c["basis"], er, None, c["prod_bcs"], mu_r=ur)
+    assert not sw.frequency_dependent
+    res = sw.solve(FREQS, precision=precision, tol=1e-8, fblock=8)
+    _check(sw, res, "tensor")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_conductive_block_reassembled_per_frequency(precision):
+    print(f"conductive block precision={precision}", flush=True)
+    c = wg_case.case()
+    m = c["mesh"]
+    T = m.n_tets
+    blk = _block(m)
+    eps_of = lambda f: _lossy_eps(T, blk, f)  # noqa: E731
+    sw = FrequencySweep(m, c["basis"], eps_of, np.ones(T, dtype=np.complex128), c["prod_bcs"])
+    assert sw.frequency_dependent
+    res = sw.solve(FREQS, precision=precision, tol=1e-8)
+    _check(sw, res, "lossy")
+    # a static sweep on the materials of the last frequency gives the same S
+    st = FrequencySweep(m, c["basis"], eps_of(FREQS[-1]), np.ones(T, dtype=np.complex128), c["prod_bcs"])
+    r2 = st.solve(FREQS[-1:], precision="fp64", tol=1e-8)
+    assert np.abs(r2.S[0] - res.S[-1]).max() <= 1e-6
+
+
+def test_refresh_needs_current_reduction():
+    """emg_system_refresh after a new pattern is a state error, not a gather
+    through the old reduced map."""
+    c = wg_case.case()
+    m, f = c["mesh"], c["basis"]
+    ctx = Context(0)
+    ctx.mesh_set(m.nodes, m.tets, f.tet_to_field, f.n_field)
+    ctx.pattern_build()
+    ctx.material_set(np.ones(m.n_tets, dtype=np.complex128), np.ones(m.n_tets, dtype=np.complex128))
+    ctx.assemble()
+    ids = np.ascontiguousarray(np.arange(f.n_field), dtype=np.int32)
+    from emerge_amd._lib import ptr
+    ctx.call("emg_system_reduce", ids.shape[0], ptr(ids))
+    ctx.call("emg_system_refresh")
+    ctx.pattern_build()
+    with pytest.raises(EmgError):
+        ctx.call("emg_system_refresh")

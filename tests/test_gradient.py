@@ -89,16 +89,24 @@ def test_gradient_matches_nodal_function_pointwise(mesh):
         assert np.abs(field - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
-def test_nodal_volume_diagonal_is_gMg(mesh):
+@pytest.mark.parametrize("kind", ["scalar", "tensor"])
+def test_nodal_volume_diagonal_is_gMg(mesh, kind):
+    """Scalar eps and a general complex (non-symmetric) 3x3 tensor per tet."""
     m, f = mesh
     T = m.n_tets
     rng = np.random.default_rng(2)
-    eps = rng.uniform(1, 4, T) * (1 - 0.01j)
+    if kind == "scalar":
+        eps = rng.uniform(1, 4, T) * (1 - 0.01j)
+        eps_t = lambda t: eps[t] * np.eye(3)  # noqa: E731
+    else:
+        eps = (np.eye(3)[:, :, None] * rng.uniform(1, 4, (3, 1, T))
+               + 0.3 * (rng.standard_normal((3, 3, T)) + 1j * rng.standard_normal((3, 3, T))))
+        eps_t = lambda t: eps[:, :, t]  # noqa: E731
     rows, cols, vals = [], [], []
     for t in range(T):
         vid = m.tets[:, t]
         _, Mt = qd.element_matrices_quadrature(m.nodes[:, vid], vid, m.edges[:, m.tet_to_edge[:, t]],
-                                               m.tris[:, m.tet_to_tri[:, t]], np.eye(3), eps[t] * np.eye(3))
+                                               m.tris[:, m.tet_to_tri[:, t]], np.eye(3), eps_t(t))
         d = f.tet_to_field[:, t]
         rows.append(np.repeat(d, 20)); cols.append(np.tile(d, 20)); vals.append(Mt.ravel())
     M = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
