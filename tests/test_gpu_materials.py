@@ -13,10 +13,14 @@ import functools
 import numpy as np
 import pytest
 
+from emerge_amd import bc as pbc
 from emerge_amd._lib import Context, EmgError
+from emerge_amd.materials import pml_layer_tensors, tet_centers
 from emerge_amd.sweep import FrequencySweep, lossy_permittivity
+from emerge_amd.workloads import WR90_A, WR90_B, waveguide
 from oracle import curlcurl as cc
 from oracle import fem
+from oracle import ports as oports
 from tests import wg_case
 
 pytestmark = pytest.mark.gpu
@@ -115,6 +119,58 @@ def test_conductive_block_reassembled_per_frequency(precision):
     st = FrequencySweep(m, c["basis"], eps_of(FREQS[-1]), np.ones(T, dtype=np.complex128), c["prod_bcs"])
     r2 = st.solve(FREQS[-1:], precision="fp64", tol=1e-8)
     assert np.abs(r2.S[0] - res.S[-1]).max() <= 1e-6
+
+
+@functools.lru_cache(maxsize=1)
+def _pml_case():
+    """WR-90, 40 mm on 11 x 5 x 16 Kuhn cells (layer boundary on a grid
+    plane), TE10 port on z = 0, a 15 mm PML (pmlbox defaults: exponent 1.5,
+    deltamax 8) beyond z = 25 mm, PEC on every other exterior triangle."""
+    L, Lm, th = 40e-3, 25e-3, 15e-3
+    p = waveguide(cells=(11, 5, 16), length=L)
+    m, f = p.mesh, p.basis
+    T = m.n_tets
+    pml = np.nonzero(tet_centers(m)[2] > Lm)[0]
+    er = np.zeros((3, 3, T), dtype=np.complex128)
+    ur = np.zeros((3, 3, T), dtype=np.complex128)
+    for k in range(3):
+        er[k, k] = 1.0
+        ur[k, k] = 1.0
+    e_p, u_p = pml_layer_tensors(m, pml, (WR90_A / 2, WR90_B / 2, Lm / 2), (WR90_A, WR90_B, Lm), (0, 0, 1), th)
+    er[:, :, pml] = e_p
+    ur[:, :, pml] = u_p
+    t1 = m.plane_tris(2, 0.0)
+    pec = np.setdiff1d(m.boundary_tris(), t1)
+    cs = ([1, 0, 0], [0, 1, 0], [0, 0, 1], [WR90_A / 2, WR90_B / 2, 0.0])
+    prod_bcs = [pbc.PEC(pec), pbc.RectangularWaveguide(t1, 1, pbc.CoordinateSystem(*cs), (WR90_A, WR90_B))]
+    o_ports = [oports.RectangularWaveguide(t1, 1, oports.CS(*cs), (WR90_A, WR90_B))]
+    E, B = cc.tet_mass_stiffness_matrices(m, f.tet_to_field, er, ur)
+    maps = dict(edge_to_field=f.edge_to_field, tri_to_field=f.tri_to_field)
+    S = {}
+    for fq in FREQS:
+        K, b, pv, ids = fem.assemble_freq_matrix(m, maps, E, B, pec, o_ports, float(fq))
+        fields = fem.solve_all_ports(K, b, pv, ids)
+        S[float(fq)] = fem.s_parameters(m, f.tet_to_field, fields, o_ports, 2 * np.pi * fq / fem.C0)
+    return dict(mesh=m, basis=f, er=er, ur=ur, prod_bcs=prod_bcs, S=S)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pml_terminated_guide_matches_superlu(precision):
+    """The guide ends in a PML instead of a port: S11 equals the oracle's
+    (tensors through the oracle's assembly, SuperLU) and stays below 0.02
+    (the layer absorbs; an inverted stretch tensor reflects |S11| = 0.27 on
+    this mesh)."""
+    print(f"pml guide precision={precision}", flush=True)
+    c = _pml_case()
+    sw = FrequencySweep(c["mesh"], c["basis"], c["er"], None, c["prod_bcs"], mu_r=c["ur"])
+    res = sw.solve(FREQS, precision=precision, tol=1e-8, fblock=4)
+    assert np.all(res.relres <= 1e-8), res.relres
+    for i, fq in enumerate(FREQS):
+        S = c["S"][float(fq)]
+        dS = np.abs(res.S[i] - S).max()
+        print(f"f={fq:.3e} |S11|={abs(res.S[i][0, 0]):.2e} dS={dS:.2e} iters={res.iters[i].tolist()}", flush=True)
+        assert dS <= 1e-6, (fq, dS)
+        assert abs(res.S[i][0, 0]) < 0.02
 
 
 def test_refresh_needs_current_reduction():
