@@ -21,6 +21,7 @@ SOURCES = [
     "emg_solve.hip",
     "emg_sparam.hip",
     "emg_topology.hip",
+    "emg_farfield.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")

@@ -1,0 +1,80 @@
+"""Far field by the Stratton-Chu integral on the GPU.
+
+Mirrors ``stratton_chu(Ein, Hin, mesh, theta, phi, k0) -> (Eout, Hout, wns)``
+(MW/sc.py:135-170, ``MW/`` = src/emerge/_emerge/physics/microwave/) with the
+sum (``stratton_chu_ff``, :41-132) in ``emg_farfield``.  The surface is
+described as the reference's ``SurfaceMesh`` describes it
+(E/mesh3d.py:690-716, 820-885): samples at the edge centres of the surface
+triangles, each triangle adding n A / 3 to the weighted normal of its three
+edges; unit normals point away from ``origin`` (``outward_normal``,
+E/mth/optimized.py:252-274) and, without an origin, away from the tet each
+triangle belongs to.  The caller samples E and H at ``edge_centers``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import Context, ptr
+
+
+class FarFieldSurface:
+    """Edge-centre samples of the triangles `tri_ids` of a Mesh3D."""
+
+    def __init__(self, mesh, tri_ids: np.ndarray, origin=None):
+        tri_ids = np.asarray(tri_ids)
+        tris = mesh.tris[:, tri_ids]
+        X = mesh.nodes[:, tris]                                     # (3, 3 vert, n)
+        cr = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], axis=0)
+        length = np.sqrt(cr[0] ** 2 + cr[1] ** 2 + cr[2] ** 2)
+        normals = cr / length
+        self.areas = length / 2
+        centres = X.mean(axis=1)
+        o = (mesh.nodes[:, np.unique(tris)].mean(axis=1) if origin is None
+             else np.asarray(origin, dtype=np.float64))
+        normals = normals * np.where(((centres - o[:, None]) * normals).sum(axis=0) < 0, -1.0, 1.0)
+        if origin is None:
+            tets = np.max(mesh.tri_to_tet[:, tri_ids], axis=0)
+            tc = mesh.nodes[:, mesh.tets[:, tets]].mean(axis=1)
+            normals = normals * np.sign(((centres - tc) * normals).sum(axis=0))
+        self.normals = normals
+        # edges (x < y) of every triangle: pairs (0,1), (1,2), (0,2)
+        pairs = np.concatenate([np.sort(tris[[0, 1]], axis=0), np.sort(tris[[1, 2]], axis=0),
+                                np.sort(tris[[0, 2]], axis=0)], axis=1)
+        self.edges, inv = np.unique(pairs, axis=1, return_inverse=True)
+        self.tri_to_edge = inv.reshape(3, -1)
+        w = normals * self.areas / 3
+        wn = np.zeros((3, self.edges.shape[1]))
+        for k in range(3):
+            wn[k] = np.bincount(self.tri_to_edge[0], w[k], self.edges.shape[1]) \
+                + np.bincount(self.tri_to_edge[1], w[k], self.edges.shape[1]) \
+                + np.bincount(self.tri_to_edge[2], w[k], self.edges.shape[1])
+        self.weighted_normals = wn
+        self.edge_centers = (mesh.nodes[:, self.edges[0]] + mesh.nodes[:, self.edges[1]]) / 2
+
+    @property
+    def exyz(self):
+        return self.edge_centers[0], self.edge_centers[1], self.edge_centers[2]
+
+
+def stratton_chu(Ein, Hin, surface: FarFieldSurface, theta, phi, k0: float, ctx: Context | None = None):
+    """Far-field E and H (complex (3, N)) in the directions (theta, phi)
+    from E and H (complex (3, M)) at the surface's edge centres; returns
+    (Eout, Hout, weighted normals) as the reference does."""
+    E = np.ascontiguousarray(Ein, dtype=np.complex128)
+    H = np.ascontiguousarray(Hin, dtype=np.complex128)
+    M = surface.edge_centers.shape[1]
+    if E.shape != (3, M) or H.shape != (3, M):
+        raise ValueError("Ein and Hin must have shape (3, n_samples)")
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    ph = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+    if th.shape != ph.shape or th.ndim != 1:
+        raise ValueError("theta and phi must be 1-D arrays of one length")
+    pos = np.ascontiguousarray(surface.edge_centers, dtype=np.float64)
+    wn = np.ascontiguousarray(surface.weighted_normals, dtype=np.float64)
+    ctx = ctx if ctx is not None else Context(0)
+    N = th.shape[0]
+    Eout = np.empty((3, N), dtype=np.complex128)
+    Hout = np.empty((3, N), dtype=np.complex128)
+    ctx.call("emg_farfield", M, ptr(E), ptr(H), ptr(pos), ptr(wn), N, ptr(th), ptr(ph), float(k0),
+             ptr(Eout), ptr(Hout))
+    return Eout, Hout, wn

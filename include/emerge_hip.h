@@ -284,6 +284,16 @@ int emg_csr_set(emg_ctx* ctx, int64_t n, const int32_t* indptr, const int32_t* i
 int emg_csr_solve(emg_ctx* ctx, int32_t nrhs, const double* b, int precision, double tol,
                   int restart, int max_cycles, double* x, double* relres_out, int64_t* iters_out);
 
+/* Stratton-Chu far field (replaces stratton_chu_ff, MW/sc.py:41-132, called
+ * by stratton_chu, :135-170): M surface samples with fields E, H (complex128
+ * [3][M], interleaved), positions pos and area-weighted normals wn (real
+ * [3][M]); N directions theta, phi (radians); wavenumber k0.  Samples with
+ * |E| <= 1e-3 max|E| are left out.  Eout, Hout: complex128 [3][N],
+ * Eout = -j k0 / (4 pi) r x sum_j [(n x E)_j - Z0 r x (n x H)_j]
+ * exp(j k0 r . x_j), Hout = r x Eout / Z0.  Host buffers; synchronous. */
+int emg_farfield(emg_ctx* ctx, int64_t M, const double* E, const double* H, const double* pos, const double* wn,
+                 int64_t N, const double* theta, const double* phi, double k0, double* Eout, double* Hout);
+
 #ifdef __cplusplus
 }
 #endif

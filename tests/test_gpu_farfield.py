@@ -1,0 +1,81 @@
+"""GPU far field (emg_farfield) against the oracle (oracle/farfield.py).
+
+Bar: |Eout - Eout_oracle| <= 1e-10 max|Eout_oracle| (FP64; the sums run in
+another order), Hout likewise; the dipole's far field also within 1e-3 of
+its closed form, and a plane wave through the closed surface radiates
+< 3e-3 of its E-term alone.
+"""
+import numpy as np
+import pytest
+
+from emerge_amd._lib import Context, EmgError, ptr
+from emerge_amd.farfield import FarFieldSurface, stratton_chu
+from oracle import farfield as ff
+from tests.test_farfield import K, cube_surface, dipole_fields
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+def _surface(n):
+    m, bt, pos, wn = cube_surface(n)
+    s = FarFieldSurface(m, bt)
+    return s
+
+
+def _compare(ctx, s, E, H, T, P, k0):
+    Eo, Ho, wn = stratton_chu(E, H, s, T, P, k0, ctx=ctx)
+    Er, Hr = ff.stratton_chu_ff(E, H, s.edge_centers, s.weighted_normals, T, P, k0)
+    scale = np.abs(Er).max()
+    assert np.abs(Eo - Er).max() <= 1e-10 * scale, np.abs(Eo - Er).max() / scale
+    assert np.abs(Ho - Hr).max() <= 1e-10 * scale / ff.Z0
+    return Eo, Ho
+
+
+def test_dipole_matches_oracle_and_closed_form(ctx):
+    s = _surface(6)
+    E, H = dipole_fields(s.edge_centers)
+    T = np.linspace(0.2, np.pi - 0.2, 9)
+    P = np.full(9, 0.7)
+    Eo, _ = _compare(ctx, s, E, H, T, P, K)
+    tho = np.stack([np.cos(T) * np.cos(P), np.cos(T) * np.sin(P), -np.sin(T)])
+    ref = 1j * ff.Z0 * K * np.sin(T) / (4 * np.pi)
+    assert np.abs((Eo * tho).sum(axis=0) - ref).max() <= 1e-3 * np.abs(ref).max()
+
+
+def test_plane_wave_extinction(ctx):
+    s = _surface(6)
+    z = s.edge_centers[2]
+    E = np.stack([np.exp(-1j * K * z), 0 * z, 0 * z])
+    H = np.stack([0 * z, np.exp(-1j * K * z) / ff.Z0, 0 * z])
+    T = np.linspace(0.1, np.pi - 0.1, 12)
+    P = np.linspace(0, 2 * np.pi, 12)
+    Eo, Ho, _ = stratton_chu(E, H, s, T, P, K, ctx=ctx)
+    Ee, _, _ = stratton_chu(E, 0 * H, s, T, P, K, ctx=ctx)
+    assert np.abs(Eo).max() < 3e-3 * np.abs(Ee).max()
+
+
+@pytest.mark.parametrize("n,ndir", [(10, 700), (3, 1)])
+def test_many_tiles_and_chunks_with_filtered_samples(ctx, n, ndir):
+    """Several direction tiles and sample chunks (n = 10: 1,800 samples,
+    700 directions), one direction; some samples under the 1e-3 level."""
+    s = _surface(n)
+    E, H = dipole_fields(s.edge_centers)
+    mag = np.sqrt((np.abs(E) ** 2).sum(axis=0))
+    low = np.arange(0, E.shape[1], 7)
+    E[:, low] *= (0.5e-3 * mag.max() / mag[low])[None, :]
+    rng = np.random.default_rng(3)
+    T = rng.uniform(0, np.pi, ndir)
+    P = rng.uniform(0, 2 * np.pi, ndir)
+    print(f"samples={E.shape[1]} directions={ndir}", flush=True)
+    _compare(ctx, s, E, H, T, P, 1.3 * K)
+
+
+def test_rejects_empty_input(ctx):
+    z = np.zeros(6)
+    with pytest.raises(EmgError):
+        ctx.call("emg_farfield", 0, ptr(z), ptr(z), ptr(z), ptr(z), 1, ptr(z), ptr(z), 1.0, ptr(z), ptr(z))
