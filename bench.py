@@ -227,9 +227,57 @@ def config_bench(name: str, nf: int, fblock: int) -> dict:
                    sequential={"seconds": round(ts, 3), "converged": ok_s, "freq_points_per_s": round(ok_s / ts, 3)},
                    concurrent={"seconds": round(tc, 3), "converged": ok_c, "freq_points_per_s": round(ok_c / tc, 3)},
                    max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))))
+    elif name == "farfield":
+        out.update(farfield_bench())
     else:
         raise ValueError(name)
     return out
+
+
+def farfield_bench(n: int = 60, reps: int = 5) -> dict:
+    """Stratton-Chu far field over the surface of the n^3 Kuhn cube (edge-
+    centre samples, synthetic complex E and H from PCG64 seed 0, all above
+    the 1e-3 level) at 1 degree over the whole sphere (181 x 361
+    directions), k0 = 2 pi / (side / 4).  Rate in (direction, sample) pairs
+    per second, wall time per call (host copies included); CPU baseline:
+    the oracle's NumPy restatement on a bounded set of directions."""
+    from emerge_amd._lib import Context
+    from emerge_amd.farfield import FarFieldSurface, stratton_chu
+    from emerge_amd.mesh import Mesh3D, kuhn_box
+    nodes, tets = kuhn_box(n, n, n, size=(1.0, 1.0, 1.0))
+    m = Mesh3D(nodes - 0.5, tets)
+    surf = FarFieldSurface(m, m.boundary_tris(), origin=(0.0, 0.0, 0.0))
+    M = surf.edge_centers.shape[1]
+    rng = np.random.default_rng(0)
+    E = (1 + rng.random((3, M))) * np.exp(2j * np.pi * rng.random((3, M)))
+    H = (1 + rng.random((3, M))) * np.exp(2j * np.pi * rng.random((3, M))) / 376.73031366857
+    T, P = np.meshgrid(np.radians(np.arange(181.0)), np.radians(np.arange(361.0)), indexing="ij")
+    T, P = T.ravel(), P.ravel()
+    k0 = 2 * np.pi / 0.25
+    ctx = Context(0)
+    Eo, _, _ = stratton_chu(E, H, surf, T, P, k0, ctx=ctx)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Eo, _, _ = stratton_chu(E, H, surf, T, P, k0, ctx=ctx)
+    dt = (time.perf_counter() - t0) / reps
+    pairs = M * T.shape[0]
+    res = {"workload": f"far field: surface of the {n}^3 Kuhn unit cube ({M} edge-centre samples), "
+                       f"{T.shape[0]} directions (1 degree), synthetic fields",
+           "samples": M, "directions": int(T.shape[0]), "ms_per_call": round(dt * 1e3, 3),
+           "pairs_per_s": pairs / dt, "flops_per_pair": 53,
+           "note": "wall time per call, host copies included; kernel time in the rocprof summary"}
+    if not os.environ.get("EMG_NO_CPU_BASELINE"):
+        from oracle import farfield as off
+        sel = np.arange(0, T.shape[0], 131)
+        t0 = time.perf_counter()
+        Er, _ = off.stratton_chu_ff(E, H, surf.edge_centers, surf.weighted_normals, T[sel], P[sel], k0)
+        tc = time.perf_counter() - t0
+        res["max_rel_diff_vs_oracle"] = float(np.abs(Eo[:, sel] - Er).max() / np.abs(Er).max())
+        res["cpu_baseline"] = {"pairs_per_s": M * sel.shape[0] / tc, "cores": 1, "kind": "port",
+                               "sample": f"{sel.shape[0]} directions x {M} samples, oracle NumPy "
+                                         f"restatement, {tc:.1f} s"}
+        res["speedup_vs_cpu"] = round(res["pairs_per_s"] / res["cpu_baseline"]["pairs_per_s"], 1)
+    return res
 
 
 def main() -> int:
@@ -248,7 +296,7 @@ def main() -> int:
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=None,
                     help="GMRES dimension / IDR matvecs per cycle / COCG iterations per cycle")
-    ap.add_argument("--config", choices=["cavity", "patch", "family"], default=None,
+    ap.add_argument("--config", choices=["cavity", "patch", "family", "farfield"], default=None,
                     help="run one full-size solve workload instead of the default bench")
     ap.add_argument("--config-points", type=int, default=None, help="frequency points of --config")
     ap.add_argument("--precond", default="aux", choices=["aux", "jacobi"],
@@ -282,7 +330,7 @@ def main() -> int:
     from emerge_amd.workloads import cube_assembly
 
     if args.config:
-        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16}[args.config]
+        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "farfield": 0}[args.config]
         fb = 8 if args.fblock == "auto" else args.fblock
         print(json.dumps(config_bench(args.config, nf, fb if args.config == "patch" else 32)),
               flush=True)
