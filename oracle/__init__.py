@@ -22,6 +22,9 @@ are absent), so the restatement is pinned independently:
 * direct solve: scipy SuperLU with the reference's options;
 * S-parameters: closed forms of an empty rectangular waveguide
   (|S21| -> 1, arg S21 = -beta L, S11 -> 0, reciprocity).
+* far field (`oracle.farfield`): the closed-form far field of a Hertzian
+  dipole from its exact near fields on a closed surface, the extinction
+  theorem for a plane wave, and the divergence theorem for the weights.
 
 Pinning status per function is listed in each module header.
 """
