@@ -22,6 +22,7 @@ SOURCES = [
     "emg_sparam.hip",
     "emg_topology.hip",
     "emg_farfield.hip",
+    "emg_fields.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")

@@ -1,0 +1,197 @@
+// Field of a solution at arbitrary points: E and curl E (the basis of
+// MWField.interpolate, MW/microwave_data.py:402-430, through
+// Nedelec2.interpolate / interpolate_curl, E/elements/nedelec2.py:68-92,
+// and ned2_tet_interp / ned2_tet_interp_curl, E/elements/ned2_interp.py:
+// 141-507).
+//
+// Owner search as the reference's: a point belongs to the LAST tet (highest
+// index) whose barycentric coordinates pass sum <= 1 + 1e-8 and each >= -1e-6,
+// so each lane scans the tets from the top and stops at the first hit.  The
+// scan is O(tets) per point, as the reference's loop over all tets is.
+//
+// In the owner tet, with F_i = 6V lam_i = a_i + b_i x + c_i y + d_i z and
+// g_i = (b_i, c_i, d_i) (tet_coefficients, ned2_interp.py:47-78), for edge
+// (e1, e2) sorted by global id with DOFs m1, m2 and length L:
+//   E += L (m1 F1 + m2 F2) (F2 g1 - F1 g2) / (216 V^3),
+//   curl E -= L (m1 F1 + m2 F2) (g1 x g2) / (72 V^3);
+// for face (i1, i2, i3) sorted by global id with DOFs m1, m2, L1 = |i1 i3|,
+// L2 = |i1 i2|:
+//   E += [-m1 L1 F2 (F3 g1 - F1 g3) + m2 L2 F3 (F2 g1 - F1 g2)] / (216 V^3),
+//   curl E += [-m1 L1 (g2 x (F3 g1 - F1 g3) + 2 F2 g3 x g1)
+//              + m2 L2 (g3 x (F2 g1 - F1 g2) + 2 F3 g2 x g1)] / (216 V^3)
+// (the reference writes the curl expanded; this is the same polynomial).
+#include <cmath>
+
+#include "emg_common.h"
+
+namespace {
+
+using namespace emg;
+
+__device__ inline void cross3(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__global__ void k_field_points(int64_t P, const double* __restrict__ pts, const double* __restrict__ xyz, int64_t V,
+                               const int32_t* __restrict__ tets, int64_t T, const int32_t* __restrict__ t2f,
+                               const double2* __restrict__ x, double2* __restrict__ Eo, double2* __restrict__ Co,
+                               int32_t* __restrict__ owner) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    const double px = pts[k], py = pts[P + k], pz = pts[2 * P + k];
+    int64_t t = T - 1;
+    for (; t >= 0; --t) {
+        const int32_t w0 = tets[t], w1 = tets[T + t], w2 = tets[2 * T + t], w3 = tets[3 * T + t];
+        const double ox = xyz[w0], oy = xyz[V + w0], oz = xyz[2 * V + w0];
+        const double e1[3] = {xyz[w1] - ox, xyz[V + w1] - oy, xyz[2 * V + w1] - oz};
+        const double e2[3] = {xyz[w2] - ox, xyz[V + w2] - oy, xyz[2 * V + w2] - oz};
+        const double e3[3] = {xyz[w3] - ox, xyz[V + w3] - oy, xyz[2 * V + w3] - oz};
+        const double r[3] = {px - ox, py - oy, pz - oz};
+        double n23[3], n31[3], n12[3];
+        cross3(e2, e3, n23);
+        cross3(e3, e1, n31);
+        cross3(e1, e2, n12);
+        const double det = e1[0] * n23[0] + e1[1] * n23[1] + e1[2] * n23[2];
+        const double l0 = (r[0] * n23[0] + r[1] * n23[1] + r[2] * n23[2]) / det;
+        const double l1 = (r[0] * n31[0] + r[1] * n31[1] + r[2] * n31[2]) / det;
+        const double l2 = (r[0] * n12[0] + r[1] * n12[1] + r[2] * n12[2]) / det;
+        if (l0 + l1 + l2 <= 1.00000001 && l0 >= -1e-6 && l1 >= -1e-6 && l2 >= -1e-6) break;
+    }
+    owner[k] = int32_t(t);
+    if (t < 0) {
+        const double nan = __builtin_nan("");
+        for (int d = 0; d < 3; ++d) {
+            Eo[d * P + k] = make_double2(nan, nan);
+            Co[d * P + k] = make_double2(nan, nan);
+        }
+        return;
+    }
+    int w[4];
+    double xv[4], yv[4], zv[4];
+    for (int j = 0; j < 4; ++j) {
+        w[j] = tets[j * T + t];
+        xv[j] = xyz[w[j]];
+        yv[j] = xyz[V + w[j]];
+        zv[j] = xyz[2 * V + w[j]];
+    }
+    const double x1 = xv[0], x2 = xv[1], x3 = xv[2], x4 = xv[3];
+    const double y1 = yv[0], y2 = yv[1], y3 = yv[2], y4 = yv[3];
+    const double z1 = zv[0], z2 = zv[1], z3 = zv[2], z4 = zv[3];
+    // cofactors (tet_coefficients, E/elements/ned2_interp.py:47-78): F_j = 6V lam_j
+    const double aa[4] = {x2*y3*z4 - x2*y4*z3 - x3*y2*z4 + x3*y4*z2 + x4*y2*z3 - x4*y3*z2,
+                          -x1*y3*z4 + x1*y4*z3 + x3*y1*z4 - x3*y4*z1 - x4*y1*z3 + x4*y3*z1,
+                          x1*y2*z4 - x1*y4*z2 - x2*y1*z4 + x2*y4*z1 + x4*y1*z2 - x4*y2*z1,
+                          -x1*y2*z3 + x1*y3*z2 + x2*y1*z3 - x2*y3*z1 - x3*y1*z2 + x3*y2*z1};
+    const double bb[4] = {-y2*z3 + y2*z4 + y3*z2 - y3*z4 - y4*z2 + y4*z3,
+                          y1*z3 - y1*z4 - y3*z1 + y3*z4 + y4*z1 - y4*z3,
+                          -y1*z2 + y1*z4 + y2*z1 - y2*z4 - y4*z1 + y4*z2,
+                          y1*z2 - y1*z3 - y2*z1 + y2*z3 + y3*z1 - y3*z2};
+    const double cc[4] = {x2*z3 - x2*z4 - x3*z2 + x3*z4 + x4*z2 - x4*z3,
+                          -x1*z3 + x1*z4 + x3*z1 - x3*z4 - x4*z1 + x4*z3,
+                          x1*z2 - x1*z4 - x2*z1 + x2*z4 + x4*z1 - x4*z2,
+                          -x1*z2 + x1*z3 + x2*z1 - x2*z3 - x3*z1 + x3*z2};
+    const double dd[4] = {-x2*y3 + x2*y4 + x3*y2 - x3*y4 - x4*y2 + x4*y3,
+                          x1*y3 - x1*y4 - x3*y1 + x3*y4 + x4*y1 - x4*y3,
+                          -x1*y2 + x1*y4 + x2*y1 - x2*y4 - x4*y1 + x4*y2,
+                          x1*y2 - x1*y3 - x2*y1 + x2*y3 + x3*y1 - x3*y2};
+    const double vol = fabs(xv[0] * bb[0] + xv[1] * bb[1] + xv[2] * bb[2] + xv[3] * bb[3]) / 6.0;
+    const double V1 = 216.0 * vol * vol * vol;
+    const double V2 = 72.0 * vol * vol * vol;
+    double Fv[4], g[4][3];
+    for (int j = 0; j < 4; ++j) {
+        Fv[j] = aa[j] + bb[j] * px + cc[j] * py + dd[j] * pz;
+        g[j][0] = bb[j];
+        g[j][1] = cc[j];
+        g[j][2] = dd[j];
+    }
+    auto dist = [&](int p, int q) {
+        return sqrt((xv[p] - xv[q]) * (xv[p] - xv[q]) + (yv[p] - yv[q]) * (yv[p] - yv[q]) +
+                    (zv[p] - zv[q]) * (zv[p] - zv[q]));
+    };
+    double2 dof[20];
+    for (int i = 0; i < 20; ++i) dof[i] = x[t2f[i * T + t]];
+    double E[3][2] = {{0, 0}, {0, 0}, {0, 0}}, C[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    const int le[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {3, 1}, {2, 3}};
+    const int lf[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 2, 3}};
+    for (int ie = 0; ie < 6; ++ie) {
+        int e1 = le[ie][0], e2 = le[ie][1];
+        if (w[e1] > w[e2]) { const int tmp = e1; e1 = e2; e2 = tmp; }
+        const double L = dist(e1, e2);
+        const double2 m1 = dof[ie], m2 = dof[ie + 10];
+        const double phr = L * (m1.x * Fv[e1] + m2.x * Fv[e2]), phi = L * (m1.y * Fv[e1] + m2.y * Fv[e2]);
+        double gx[3];
+        cross3(g[e1], g[e2], gx);
+        for (int d = 0; d < 3; ++d) {
+            const double u = (g[e1][d] * Fv[e2] - g[e2][d] * Fv[e1]) / V1;
+            E[d][0] += phr * u;
+            E[d][1] += phi * u;
+            C[d][0] -= phr * gx[d] / V2;
+            C[d][1] -= phi * gx[d] / V2;
+        }
+    }
+    for (int it = 0; it < 4; ++it) {
+        int v[3] = {lf[it][0], lf[it][1], lf[it][2]};
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2 - a; ++b)
+                if (w[v[b]] > w[v[b + 1]]) { const int tmp = v[b]; v[b] = v[b + 1]; v[b + 1] = tmp; }
+        const int i1 = v[0], i2 = v[1], i3 = v[2];
+        const double L1 = dist(i1, i3), L2 = dist(i1, i2);
+        const double2 m1 = dof[it + 6], m2 = dof[it + 16];
+        const double F1 = Fv[i1], F2 = Fv[i2], F3 = Fv[i3];
+        double a13[3], a12[3];
+        for (int d = 0; d < 3; ++d) {
+            a13[d] = F3 * g[i1][d] - F1 * g[i3][d];
+            a12[d] = F2 * g[i1][d] - F1 * g[i2][d];
+        }
+        double c1[3], c2[3], x31[3], x21[3];
+        cross3(g[i2], a13, c1);
+        cross3(g[i3], a12, c2);
+        cross3(g[i3], g[i1], x31);
+        cross3(g[i2], g[i1], x21);
+        for (int d = 0; d < 3; ++d) {
+            const double t1 = c1[d] + 2.0 * F2 * x31[d];
+            const double t2 = c2[d] + 2.0 * F3 * x21[d];
+            // -m1 L1 (F2 a13, t1) + m2 L2 (F3 a12, t2), over V1
+            E[d][0] += (-m1.x * L1 * F2 * a13[d] + m2.x * L2 * F3 * a12[d]) / V1;
+            E[d][1] += (-m1.y * L1 * F2 * a13[d] + m2.y * L2 * F3 * a12[d]) / V1;
+            C[d][0] += (-m1.x * L1 * t1 + m2.x * L2 * t2) / V1;
+            C[d][1] += (-m1.y * L1 * t1 + m2.y * L2 * t2) / V1;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        Eo[d * P + k] = make_double2(E[d][0], E[d][1]);
+        Co[d * P + k] = make_double2(C[d][0], C[d][1]);
+    }
+}
+
+}  // namespace
+
+extern "C" int emg_field_points(emg_ctx* c, const double* field, int64_t P, const double* pts, double* E,
+                                double* curlE, int32_t* owner) {
+    EMG_CHECK_CTX(c);
+    if (c->n_tets == 0) return fail(c, EMG_ERR_STATE, "emg_field_points: no mesh");
+    if (P < 1) return fail(c, EMG_ERR_ARG, "emg_field_points: no points");
+    if (!field || !pts || !E || !curlE || !owner) return fail(c, EMG_ERR_ARG, "emg_field_points: null array");
+    hipStream_t st = c->stream;
+    const int64_t N = c->n_field;
+    DBuf<double> dx, dp, dE, dC;
+    DBuf<int32_t> dow;
+    EMG_HIP(c, dx.alloc(2 * N));
+    EMG_HIP(c, dp.alloc(3 * P));
+    EMG_HIP(c, dE.alloc(6 * P));
+    EMG_HIP(c, dC.alloc(6 * P));
+    EMG_HIP(c, dow.alloc(P));
+    EMG_HIP(c, hipMemcpyAsync(dx.p, field, sizeof(double) * 2 * N, hipMemcpyHostToDevice, st));
+    EMG_HIP(c, hipMemcpyAsync(dp.p, pts, sizeof(double) * 3 * P, hipMemcpyHostToDevice, st));
+    k_field_points<<<unsigned((P + 127) / 128), 128, 0, st>>>(
+        P, dp.p, c->xyz.p, c->n_nodes, c->tets.p, c->n_tets, c->t2f.p, reinterpret_cast<const double2*>(dx.p),
+        reinterpret_cast<double2*>(dE.p), reinterpret_cast<double2*>(dC.p), dow.p);
+    EMG_HIP(c, hipGetLastError());
+    EMG_HIP(c, hipMemcpyAsync(E, dE.p, sizeof(double) * 6 * P, hipMemcpyDeviceToHost, st));
+    EMG_HIP(c, hipMemcpyAsync(curlE, dC.p, sizeof(double) * 6 * P, hipMemcpyDeviceToHost, st));
+    EMG_HIP(c, hipMemcpyAsync(owner, dow.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, st));
+    EMG_HIP(c, hipStreamSynchronize(st));
+    return EMG_OK;
+}

@@ -186,6 +186,35 @@ class FrequencySweep:
             self._aux_on = False
             self._set_aux(True)
 
+    def interpolate(self, field: np.ndarray, xs, ys, zs, freq: float, mu_r=None):
+        """E and H of a full-length solution (``SweepResult.fields``) at the
+        points (xs, ys, zs), as MWField.interpolate (MW/microwave_data.py:
+        402-430): E interpolated in the last tet containing each point, H =
+        curl E / (-j w mu0 mu_r) with the owner tet's scalar mu_r (default:
+        the sweep's isotropic 1 / inv_mu_r; pass per-tet scalars for tensor
+        or frequency-dependent media).  NaN outside the mesh.  Returns
+        (E, H), complex (3, P)."""
+        pts = np.ascontiguousarray(np.vstack([np.ravel(xs), np.ravel(ys), np.ravel(zs)]), dtype=np.float64)
+        P = pts.shape[1]
+        x = np.ascontiguousarray(field, dtype=np.complex128)
+        if x.shape != (self.basis.n_field,):
+            raise ValueError("field must hold every DOF (n_field,)")
+        E = np.empty((3, P), dtype=np.complex128)
+        C = np.empty((3, P), dtype=np.complex128)
+        own = np.empty(P, dtype=np.int32)
+        self.ctx.call("emg_field_points", ptr(x), P, ptr(pts), ptr(E), ptr(C), ptr(own))
+        if mu_r is None:
+            inu = self.materials_at(freq)[1]
+            if inu is None or np.ndim(inu) != 1:
+                raise ValueError("pass mu_r (per-tet scalars) for tensor permeability")
+            mu_r = 1.0 / np.asarray(inu, dtype=np.complex128)
+        mu_r = np.asarray(mu_r, dtype=np.complex128)
+        mu0 = 1 / (C0 * C0 * EPS0)
+        ok = own >= 0
+        H = np.full((3, P), np.nan + 1j * np.nan)
+        H[:, ok] = C[:, ok] / (-1j * 2 * np.pi * freq * mu0 * mu_r[own[ok]])[None, :]
+        return E, H
+
     # ------------------------------------------------------------------ setup
     def pec_dofs(self) -> np.ndarray:
         """DOFs removed by PEC faces: both DOFs of every edge of a PEC triangle

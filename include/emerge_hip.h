@@ -284,6 +284,17 @@ int emg_csr_set(emg_ctx* ctx, int64_t n, const int32_t* indptr, const int32_t* i
 int emg_csr_solve(emg_ctx* ctx, int32_t nrhs, const double* b, int precision, double tol,
                   int restart, int max_cycles, double* x, double* relres_out, int64_t* iters_out);
 
+/* E and curl E of a solution at points (the field sampling of
+ * MWField.interpolate, MW/microwave_data.py:402-430: ned2_tet_interp and
+ * ned2_tet_interp_curl, E/elements/ned2_interp.py:141-507) on the context's
+ * mesh: field = complex128 [n_field] (all DOFs, eliminated ones 0), pts =
+ * real [3][P].  Each point belongs to the last tet that contains it
+ * (tolerances 1e-8 on the sum, 1e-6 per barycentric coordinate); E and curlE
+ * (complex128 [3][P]) are NaN and owner -1 where no tet does.  H follows as
+ * curlE / (-j w mu0 mu_r[owner]).  Host buffers; synchronous. */
+int emg_field_points(emg_ctx* ctx, const double* field, int64_t P, const double* pts, double* E, double* curlE,
+                     int32_t* owner);
+
 /* Stratton-Chu far field (replaces stratton_chu_ff, MW/sc.py:41-132, called
  * by stratton_chu, :135-170): M surface samples with fields E, H (complex128
  * [3][M], interleaved), positions pos and area-weighted normals wn (real

@@ -163,6 +163,68 @@ def ned2_tet_interp(coords, solution, mesh, tet_to_field, tetids):
     return Ex, Ey, Ez
 
 
+def ned2_tet_interp_curl(coords, solution, mesh, tet_to_field, tetids, c):
+    """E/elements/ned2_interp.py:302-507: c[tet] times the curl of the field
+    at points, each point owned by the last listed tet that contains it (the
+    same search as ``ned2_tet_interp``).  With F_i = 6V lam_i (cofactors
+    a_i + b_i x + c_i y + d_i z, gradients g_i) the reference's expanded
+    expressions are the analytic curl of its interpolation, written here in
+    product form: edge (e1, e2):  -L (m1 F1 + m2 F2) (g1 x g2) / (72 V^3);
+    face (i1, i2, i3):  [-m1 L1 (g2 x (F3 g1 - F1 g3) + 2 F2 g3 x g1)
+    + m2 L2 (g3 x (F2 g1 - F1 g2) + 2 F3 g2 x g1)] / (216 V^3).
+    Pinned by central differences of ``ned2_tet_interp`` (exact for the
+    quadratic field up to rounding)."""
+    n = coords.shape[1]
+    out = np.zeros((3, n), dtype=np.complex128)
+    found = np.zeros(n, dtype=bool)
+    assigned = np.zeros(n, dtype=np.int64) - 1
+    nodes, tets = mesh.nodes, mesh.tets
+    for itet in tetids:
+        v = nodes[:, tets[:, itet]]
+        blocal = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=1)
+        loc = np.linalg.inv(blocal) @ (coords - v[:, 0:1])
+        inside = ((loc[0] + loc[1] + loc[2]) <= 1.00000001) & (loc[0] >= -1e-6) & (loc[1] >= -1e-6) \
+            & (loc[2] >= -1e-6)
+        assigned[inside] = itet
+    nE = mesh.n_edges
+    for itet in tetids:
+        inside = assigned == itet
+        if not inside.any():
+            continue
+        P = coords[:, inside]
+        xv, yv, zv = nodes[0, tets[:, itet]], nodes[1, tets[:, itet]], nodes[2, tets[:, itet]]
+        a_s, b_s, c_s, d_s, V = tet_coefficients(xv, yv, zv)
+        g = np.stack([b_s, c_s, d_s])                                  # (3, 4)
+        Fv = a_s[:, None] + g.T @ P                                    # (4, m)
+        le = local_mapping(tets[:, itet][:, None], mesh.edges[:, tet_to_field[:6, itet]][:, :, None])[:, :, 0]
+        lt = local_mapping(tets[:, itet][:, None], mesh.tris[:, tet_to_field[6:10, itet] - nE][:, :, None])[:, :, 0]
+        Et = solution[tet_to_field[:, itet]]
+        V1 = 216 * V ** 3
+        V2 = 72 * V ** 3
+        acc = np.zeros((3, P.shape[1]), dtype=np.complex128)
+
+        def dist(p, q):
+            return np.sqrt((xv[p] - xv[q]) ** 2 + (yv[p] - yv[q]) ** 2 + (zv[p] - zv[q]) ** 2)
+
+        for ie in range(6):
+            e1, e2 = le[:, ie]
+            L = dist(e1, e2)
+            phi = L * (Et[ie] * Fv[e1] + Et[ie + 10] * Fv[e2])
+            acc -= np.cross(g[:, e1], g[:, e2])[:, None] * phi[None, :] / V2
+        for it in range(4):
+            i1, i2, i3 = lt[:, it]
+            L1, L2 = dist(i1, i3), dist(i1, i2)
+            g1, g2, g3 = g[:, i1][:, None], g[:, i2][:, None], g[:, i3][:, None]
+            F1, F2, F3 = Fv[i1][None, :], Fv[i2][None, :], Fv[i3][None, :]
+            t1 = np.cross(g2, F3 * g1 - F1 * g3, axis=0) + 2 * F2 * np.cross(g3, g1, axis=0)
+            t2 = np.cross(g3, F2 * g1 - F1 * g2, axis=0) + 2 * F3 * np.cross(g2, g1, axis=0)
+            acc += (-Et[it + 6] * L1 * t1 + Et[it + 16] * L2 * t2) / V1
+        out[:, inside] = acc * c[itet]
+        found[inside] = True
+    out[:, ~found] = np.nan
+    return out
+
+
 def surface_integral(nodes, tri_vertices, function, constants=None, gq_order=4):
     """E/mth/integrals.py:28-92."""
     if constants is None:
