@@ -79,3 +79,37 @@ def test_rejects_empty_input(ctx):
     z = np.zeros(6)
     with pytest.raises(EmgError):
         ctx.call("emg_farfield", 0, ptr(z), ptr(z), ptr(z), ptr(z), 1, ptr(z), ptr(z), 1.0, ptr(z), ptr(z))
+
+
+def test_aperture_far_field_from_a_solved_sweep():
+    """The whole chain on a solution: WR-90 fed on port 1, ended by a
+    first-order absorbing boundary; E and H sampled on the aperture by the
+    device (FrequencySweep.interpolate) and summed by emg_farfield, against
+    the oracle's interpolation (owner rule included) and Stratton-Chu."""
+    from emerge_amd.sweep import C0, EPS0, FrequencySweep
+    from oracle import fem
+    from tests import wg_case
+    c = wg_case.abc_case(1)
+    m, f, p = c["mesh"], c["basis"], c["problem"]
+    sw = FrequencySweep(m, f, p.eps_r, p.inv_mu_r, c["prod_bcs"])
+    freq = 10e9
+    k0 = 2 * np.pi * freq / C0
+    res = sw.solve([freq], precision="fp64", tol=1e-10)
+    x = res.fields[(0, 1)]
+    surf = FarFieldSurface(m, c["abc_tris"])
+    assert np.all(surf.weighted_normals[2] > 0)      # aperture normals point out of the guide (+z)
+    E, H = sw.interpolate(x, *surf.exyz, freq=freq)
+    T = np.linspace(0.05, np.pi / 2, 40)
+    P = np.full(40, 0.3)
+    Eo, Ho, _ = stratton_chu(E, H, surf, T, P, k0)
+    allt = np.arange(m.n_tets)
+    pts = surf.edge_centers
+    Er = np.array(fem.ned2_tet_interp(pts, x, m, f.tet_to_field, allt))
+    mu0 = 1 / (C0 * C0 * EPS0)
+    Hr = fem.ned2_tet_interp_curl(pts, x, m, f.tet_to_field, allt, 1 / (-1j * 2 * np.pi * freq * mu0 / p.inv_mu_r))
+    Efr, Hfr = ff.stratton_chu_ff(Er, Hr, pts, surf.weighted_normals, T, P, k0)
+    scale = np.abs(Efr).max()
+    print(f"aperture samples={pts.shape[1]} |Eff|max={scale:.3e}", flush=True)
+    assert scale > 0
+    assert np.abs(Eo - Efr).max() <= 1e-10 * scale, np.abs(Eo - Efr).max() / scale
+    assert np.abs(Ho - Hfr).max() <= 1e-10 * scale / ff.Z0
