@@ -137,6 +137,16 @@ struct emg_ctx {
     emg::DBuf<uint64_t> topo_ekeys, topo_tkeys;
     emg::DBuf<int32_t> topo_edges, topo_tris, topo_t2e, topo_t2t, topo_r2e, topo_r2t;
 
+    // uniform grid over the tets for point location (emg_field_points),
+    // built on first use per mesh: cell c lists its tets in ascending order
+    // at grid_tets[grid_ptr[c] .. grid_ptr[c + 1])
+    bool grid_ready = false;
+    double grid_lo[3] = {0, 0, 0};
+    double grid_h = 1.0;
+    int32_t grid_n[3] = {1, 1, 1};
+    emg::DBuf<int64_t> grid_ptr;
+    emg::DBuf<int32_t> grid_tets;
+
     // Krylov method of the solves (emg_solver_config)
     int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part
     int idr_s = 4;

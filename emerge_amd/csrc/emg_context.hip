@@ -95,6 +95,7 @@ int emg_mesh_set(emg_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_tets,
     c->have_material = false;
     c->have_values = false;
     c->have_system = false;
+    c->grid_ready = false;
     return EMG_OK;
 }
 

@@ -4,10 +4,13 @@
 // and ned2_tet_interp / ned2_tet_interp_curl, E/elements/ned2_interp.py:
 // 141-507).
 //
-// Owner search as the reference's: a point belongs to the LAST tet (highest
-// index) whose barycentric coordinates pass sum <= 1 + 1e-8 and each >= -1e-6,
-// so each lane scans the tets from the top and stops at the first hit.  The
-// scan is O(tets) per point, as the reference's loop over all tets is.
+// Owner rule as the reference's: a point belongs to the LAST tet (highest
+// index) whose barycentric coordinates pass sum <= 1 + 1e-8 and each >= -1e-6.
+// A uniform grid (built once per mesh) lists in every cell, in ascending
+// order, the tets whose bounding box -- widened by 1e-5 of the tet's size,
+// beyond the tolerances -- meets the cell; every tet that can contain a point
+// is in the point's cell, so a lane scans that list from its end and stops
+// at the first hit.
 //
 // In the owner tet, with F_i = 6V lam_i = a_i + b_i x + c_i y + d_i z and
 // g_i = (b_i, c_i, d_i) (tet_coefficients, ned2_interp.py:47-78), for edge
@@ -20,7 +23,11 @@
 //   curl E += [-m1 L1 (g2 x (F3 g1 - F1 g3) + 2 F2 g3 x g1)
 //              + m2 L2 (g3 x (F2 g1 - F1 g2) + 2 F3 g2 x g1)] / (216 V^3)
 // (the reference writes the curl expanded; this is the same polynomial).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "emg_common.h"
 
@@ -34,30 +41,120 @@ __device__ inline void cross3(const double* a, const double* b, double* o) {
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+__device__ inline bool in_tet(int64_t t, double px, double py, double pz, const double* __restrict__ xyz,
+                              int64_t V, const int32_t* __restrict__ tets, int64_t T) {
+    const int32_t w0 = tets[t], w1 = tets[T + t], w2 = tets[2 * T + t], w3 = tets[3 * T + t];
+    const double ox = xyz[w0], oy = xyz[V + w0], oz = xyz[2 * V + w0];
+    const double e1[3] = {xyz[w1] - ox, xyz[V + w1] - oy, xyz[2 * V + w1] - oz};
+    const double e2[3] = {xyz[w2] - ox, xyz[V + w2] - oy, xyz[2 * V + w2] - oz};
+    const double e3[3] = {xyz[w3] - ox, xyz[V + w3] - oy, xyz[2 * V + w3] - oz};
+    const double r[3] = {px - ox, py - oy, pz - oz};
+    double n23[3], n31[3], n12[3];
+    cross3(e2, e3, n23);
+    cross3(e3, e1, n31);
+    cross3(e1, e2, n12);
+    const double det = e1[0] * n23[0] + e1[1] * n23[1] + e1[2] * n23[2];
+    const double l0 = (r[0] * n23[0] + r[1] * n23[1] + r[2] * n23[2]) / det;
+    const double l1 = (r[0] * n31[0] + r[1] * n31[1] + r[2] * n31[2]) / det;
+    const double l2 = (r[0] * n12[0] + r[1] * n12[1] + r[2] * n12[2]) / det;
+    return l0 + l1 + l2 <= 1.00000001 && l0 >= -1e-6 && l1 >= -1e-6 && l2 >= -1e-6;
+}
+
+struct Grid {
+    double lo[3];
+    double h;
+    int32_t n[3];
+};
+
+__device__ inline int cell_axis(const Grid& g, int a, double v) {
+    const double q = floor((v - g.lo[a]) / g.h);
+    return q < 0.0 ? 0 : (q >= double(g.n[a]) ? g.n[a] - 1 : int(q));
+}
+
+// cells covered by tet t's widened bounding box: per-axis ranges
+__device__ inline void tet_cells(const Grid& g, int64_t t, const double* __restrict__ xyz, int64_t V,
+                                 const int32_t* __restrict__ tets, int64_t T, int* c0, int* c1) {
+    double mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = 1e300;
+        mx[a] = -1e300;
+    }
+    for (int j = 0; j < 4; ++j) {
+        const int32_t w = tets[j * T + t];
+        for (int a = 0; a < 3; ++a) {
+            const double v = xyz[a * V + w];
+            mn[a] = fmin(mn[a], v);
+            mx[a] = fmax(mx[a], v);
+        }
+    }
+    const double size = fmax(mx[0] - mn[0], fmax(mx[1] - mn[1], mx[2] - mn[2]));
+    const double pad = 1e-5 * size;
+    for (int a = 0; a < 3; ++a) {
+        c0[a] = cell_axis(g, a, mn[a] - pad);
+        c1[a] = cell_axis(g, a, mx[a] + pad);
+    }
+}
+
+__global__ void k_grid_count(int64_t T, Grid g, const double* __restrict__ xyz, int64_t V,
+                             const int32_t* __restrict__ tets, int64_t* __restrict__ count) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int c0[3], c1[3];
+    tet_cells(g, t, xyz, V, tets, T, c0, c1);
+    count[t] = int64_t(c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1);
+}
+
+__global__ void k_grid_fill(int64_t T, Grid g, const double* __restrict__ xyz, int64_t V,
+                            const int32_t* __restrict__ tets, const int64_t* __restrict__ off,
+                            uint32_t* __restrict__ key, int32_t* __restrict__ val) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int c0[3], c1[3];
+    tet_cells(g, t, xyz, V, tets, T, c0, c1);
+    int64_t o = off[t];
+    for (int z = c0[2]; z <= c1[2]; ++z)
+        for (int y = c0[1]; y <= c1[1]; ++y)
+            for (int x = c0[0]; x <= c1[0]; ++x) {
+                key[o] = uint32_t((int64_t(z) * g.n[1] + y) * g.n[0] + x);
+                val[o] = int32_t(t);
+                ++o;
+            }
+}
+
+// ptr[c] = first sorted entry of cell c (lower bound), ptr[ncells] = total
+__global__ void k_grid_ptr(int64_t ncells, const uint32_t* __restrict__ key, int64_t total,
+                           int64_t* __restrict__ ptr) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c > ncells) return;
+    if (c == ncells) {
+        ptr[c] = total;
+        return;
+    }
+    int64_t lo = 0, hi = total;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (int64_t(key[mid]) < c) lo = mid + 1;
+        else hi = mid;
+    }
+    ptr[c] = lo;
+}
+
 __global__ void k_field_points(int64_t P, const double* __restrict__ pts, const double* __restrict__ xyz, int64_t V,
                                const int32_t* __restrict__ tets, int64_t T, const int32_t* __restrict__ t2f,
+                               Grid grid, const int64_t* __restrict__ gptr, const int32_t* __restrict__ gtets,
                                const double2* __restrict__ x, double2* __restrict__ Eo, double2* __restrict__ Co,
                                int32_t* __restrict__ owner) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= P) return;
     const double px = pts[k], py = pts[P + k], pz = pts[2 * P + k];
-    int64_t t = T - 1;
-    for (; t >= 0; --t) {
-        const int32_t w0 = tets[t], w1 = tets[T + t], w2 = tets[2 * T + t], w3 = tets[3 * T + t];
-        const double ox = xyz[w0], oy = xyz[V + w0], oz = xyz[2 * V + w0];
-        const double e1[3] = {xyz[w1] - ox, xyz[V + w1] - oy, xyz[2 * V + w1] - oz};
-        const double e2[3] = {xyz[w2] - ox, xyz[V + w2] - oy, xyz[2 * V + w2] - oz};
-        const double e3[3] = {xyz[w3] - ox, xyz[V + w3] - oy, xyz[2 * V + w3] - oz};
-        const double r[3] = {px - ox, py - oy, pz - oz};
-        double n23[3], n31[3], n12[3];
-        cross3(e2, e3, n23);
-        cross3(e3, e1, n31);
-        cross3(e1, e2, n12);
-        const double det = e1[0] * n23[0] + e1[1] * n23[1] + e1[2] * n23[2];
-        const double l0 = (r[0] * n23[0] + r[1] * n23[1] + r[2] * n23[2]) / det;
-        const double l1 = (r[0] * n31[0] + r[1] * n31[1] + r[2] * n31[2]) / det;
-        const double l2 = (r[0] * n12[0] + r[1] * n12[1] + r[2] * n12[2]) / det;
-        if (l0 + l1 + l2 <= 1.00000001 && l0 >= -1e-6 && l1 >= -1e-6 && l2 >= -1e-6) break;
+    const int64_t cell = (int64_t(cell_axis(grid, 2, pz)) * grid.n[1] + cell_axis(grid, 1, py)) * grid.n[0] +
+                         cell_axis(grid, 0, px);
+    int64_t t = -1;
+    for (int64_t q = gptr[cell + 1] - 1; q >= gptr[cell]; --q) {
+        if (in_tet(gtets[q], px, py, pz, xyz, V, tets, T)) {
+            t = gtets[q];
+            break;
+        }
     }
     owner[k] = int32_t(t);
     if (t < 0) {
@@ -166,6 +263,85 @@ __global__ void k_field_points(int64_t P, const double* __restrict__ pts, const 
     }
 }
 
+// Grid of about one tet per cell (edge h = cbrt(6 vol / T)), at most 1024
+// cells per axis and 2^26 cells; points and tets use the same clamped cell
+// index, so points outside the grid land in a border cell and find no tet.
+static int build_grid(emg_ctx* c) {
+    hipStream_t st = c->stream;
+    const int64_t V = c->n_nodes, T = c->n_tets;
+    std::vector<double> h(3 * V);
+    EMG_HIP(c, hipMemcpyAsync(h.data(), c->xyz.p, sizeof(double) * 3 * V, hipMemcpyDeviceToHost, st));
+    EMG_HIP(c, hipStreamSynchronize(st));
+    Grid g;
+    double ext[3];
+    for (int a = 0; a < 3; ++a) {
+        const auto mm = std::minmax_element(h.begin() + a * V, h.begin() + (a + 1) * V);
+        g.lo[a] = *mm.first;
+        ext[a] = *mm.second - *mm.first;
+    }
+    const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
+    double vol = 1.0;
+    for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], 1e-9 * emax);
+    double cs = std::cbrt(6.0 * vol / double(T));
+    if (!(cs > 0.0)) cs = emax > 0.0 ? emax : 1.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        int64_t cells = 1;
+        for (int a = 0; a < 3; ++a) {
+            g.n[a] = int32_t(std::min<double>(1024.0, std::max(1.0, std::floor(ext[a] / cs) + 1.0)));
+            cells *= g.n[a];
+        }
+        if (cells <= (int64_t(1) << 26)) break;
+        cs *= std::cbrt(double(cells) / double(int64_t(1) << 26)) * 1.01;
+    }
+    g.h = cs;
+    const int64_t ncells = int64_t(g.n[0]) * g.n[1] * g.n[2];
+    DBuf<int64_t> cnt, off;
+    EMG_HIP(c, cnt.alloc(T));
+    EMG_HIP(c, off.alloc(T));
+    const unsigned bt = unsigned((T + 255) / 256);
+    k_grid_count<<<bt, 256, 0, st>>>(T, g, c->xyz.p, V, c->tets.p, cnt.p);
+    EMG_HIP(c, hipGetLastError());
+    size_t sb = 0;
+    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cnt.p, off.p, int(T), st));
+    DBuf<unsigned char> tmp;
+    EMG_HIP(c, tmp.alloc(std::max<size_t>(sb, 1)));
+    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp.p, sb, cnt.p, off.p, int(T), st));
+    int64_t last_off = 0, last_cnt = 0;
+    EMG_HIP(c, hipMemcpyAsync(&last_off, off.p + (T - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    EMG_HIP(c, hipMemcpyAsync(&last_cnt, cnt.p + (T - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    EMG_HIP(c, hipStreamSynchronize(st));
+    const int64_t total = last_off + last_cnt;
+    if (total >= (int64_t(1) << 31))
+        return fail(c, EMG_ERR_ARG, "emg_field_points: point grid too large (%lld entries)", (long long)total);
+    DBuf<uint32_t> k_in, k_out;
+    DBuf<int32_t> v_in;
+    EMG_HIP(c, k_in.alloc(total));
+    EMG_HIP(c, k_out.alloc(total));
+    EMG_HIP(c, v_in.alloc(total));
+    EMG_HIP(c, c->grid_tets.alloc(total));
+    k_grid_fill<<<bt, 256, 0, st>>>(T, g, c->xyz.p, V, c->tets.p, off.p, k_in.p, v_in.p);
+    EMG_HIP(c, hipGetLastError());
+    int bits = 1;
+    while ((int64_t(1) << bits) < ncells) ++bits;
+    size_t rb = 0;
+    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, rb, k_in.p, k_out.p, v_in.p, c->grid_tets.p, int(total), 0,
+                                                  bits, st));
+    EMG_HIP(c, tmp.alloc(std::max<size_t>(rb, 1)));
+    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp.p, rb, k_in.p, k_out.p, v_in.p, c->grid_tets.p, int(total), 0,
+                                                  bits, st));
+    EMG_HIP(c, c->grid_ptr.alloc(ncells + 1));
+    k_grid_ptr<<<unsigned((ncells + 1 + 255) / 256), 256, 0, st>>>(ncells, k_out.p, total, c->grid_ptr.p);
+    EMG_HIP(c, hipGetLastError());
+    EMG_HIP(c, hipStreamSynchronize(st));
+    for (int a = 0; a < 3; ++a) {
+        c->grid_lo[a] = g.lo[a];
+        c->grid_n[a] = g.n[a];
+    }
+    c->grid_h = g.h;
+    c->grid_ready = true;
+    return EMG_OK;
+}
+
 }  // namespace
 
 extern "C" int emg_field_points(emg_ctx* c, const double* field, int64_t P, const double* pts, double* E,
@@ -174,6 +350,16 @@ extern "C" int emg_field_points(emg_ctx* c, const double* field, int64_t P, cons
     if (c->n_tets == 0) return fail(c, EMG_ERR_STATE, "emg_field_points: no mesh");
     if (P < 1) return fail(c, EMG_ERR_ARG, "emg_field_points: no points");
     if (!field || !pts || !E || !curlE || !owner) return fail(c, EMG_ERR_ARG, "emg_field_points: null array");
+    if (!c->grid_ready) {
+        const int rc = build_grid(c);
+        if (rc != EMG_OK) return rc;
+    }
+    Grid g;
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = c->grid_lo[a];
+        g.n[a] = c->grid_n[a];
+    }
+    g.h = c->grid_h;
     hipStream_t st = c->stream;
     const int64_t N = c->n_field;
     DBuf<double> dx, dp, dE, dC;
@@ -186,8 +372,9 @@ extern "C" int emg_field_points(emg_ctx* c, const double* field, int64_t P, cons
     EMG_HIP(c, hipMemcpyAsync(dx.p, field, sizeof(double) * 2 * N, hipMemcpyHostToDevice, st));
     EMG_HIP(c, hipMemcpyAsync(dp.p, pts, sizeof(double) * 3 * P, hipMemcpyHostToDevice, st));
     k_field_points<<<unsigned((P + 127) / 128), 128, 0, st>>>(
-        P, dp.p, c->xyz.p, c->n_nodes, c->tets.p, c->n_tets, c->t2f.p, reinterpret_cast<const double2*>(dx.p),
-        reinterpret_cast<double2*>(dE.p), reinterpret_cast<double2*>(dC.p), dow.p);
+        P, dp.p, c->xyz.p, c->n_nodes, c->tets.p, c->n_tets, c->t2f.p, g, c->grid_ptr.p, c->grid_tets.p,
+        reinterpret_cast<const double2*>(dx.p), reinterpret_cast<double2*>(dE.p), reinterpret_cast<double2*>(dC.p),
+        dow.p);
     EMG_HIP(c, hipGetLastError());
     EMG_HIP(c, hipMemcpyAsync(E, dE.p, sizeof(double) * 6 * P, hipMemcpyDeviceToHost, st));
     EMG_HIP(c, hipMemcpyAsync(curlE, dC.p, sizeof(double) * 6 * P, hipMemcpyDeviceToHost, st));

@@ -42,3 +42,28 @@ def test_fields_at_points_match_oracle():
         err = np.abs(a[:, :n_in] - b[:, :n_in]).max() / scale
         print(f"rel err {err:.2e}", flush=True)
         assert err <= 1e-11
+
+
+def test_point_grid_follows_a_new_mesh():
+    """The point grid is built per mesh: after emg_mesh_set with a shifted
+    copy of the mesh on the same context, owners and fields follow it."""
+    from emerge_amd._lib import Context, ptr
+    c = wg_case.case()
+    m, f = c["mesh"], c["basis"]
+    ctx = Context(0)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(f.n_field) + 1j * rng.standard_normal(f.n_field)
+    pts = np.ascontiguousarray(rng.random((3, 50)) * np.array([[WR90_A], [WR90_B], [wg_case.L]]))
+    out = []
+    for shift in (0.0, 0.05):
+        ctx.mesh_set(m.nodes + np.array([[0.0], [0.0], [shift]]), m.tets, f.tet_to_field, f.n_field)
+        E = np.empty((3, 50), dtype=np.complex128)
+        C = np.empty((3, 50), dtype=np.complex128)
+        own = np.empty(50, dtype=np.int32)
+        q = np.ascontiguousarray(pts + np.array([[0.0], [0.0], [shift]]))
+        ctx.call("emg_field_points", ptr(x), 50, ptr(q), ptr(E), ptr(C), ptr(own))
+        out.append((E, C, own))
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    assert np.all(out[0][2] >= 0)
+    for a, b in ((out[0][0], out[1][0]), (out[0][1], out[1][1])):
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
