@@ -238,10 +238,13 @@ def farfield_bench(n: int = 60, reps: int = 5) -> dict:
     """Stratton-Chu far field over the surface of the n^3 Kuhn cube (edge-
     centre samples, synthetic complex E and H from PCG64 seed 0, all above
     the 1e-3 level) at 1 degree over the whole sphere (181 x 361
-    directions), k0 = 2 pi / (side / 4).  Rate in (direction, sample) pairs
+    directions), k0 = 2 pi / (side / 4); before it, E and curl E of a random
+    solution sampled at the same points on the cube's mesh (point grid built
+    by the first call).  Rate in (direction, sample) pairs
     per second, wall time per call (host copies included); CPU baseline:
     the oracle's NumPy restatement on a bounded set of directions."""
-    from emerge_amd._lib import Context
+    from emerge_amd._lib import Context, ptr
+    from emerge_amd.basis import Nedelec2
     from emerge_amd.farfield import FarFieldSurface, stratton_chu
     from emerge_amd.mesh import Mesh3D, kuhn_box
     nodes, tets = kuhn_box(n, n, n, size=(1.0, 1.0, 1.0))
@@ -249,6 +252,21 @@ def farfield_bench(n: int = 60, reps: int = 5) -> dict:
     surf = FarFieldSurface(m, m.boundary_tris(), origin=(0.0, 0.0, 0.0))
     M = surf.edge_centers.shape[1]
     rng = np.random.default_rng(0)
+    # field sampling at the surface samples (emg_field_points) from a random
+    # solution vector on the same mesh: first call builds the point grid
+    f = Nedelec2(m)
+    fctx = Context(0)
+    fctx.mesh_set(m.nodes, m.tets, f.tet_to_field, f.n_field)
+    xs = rng.standard_normal(f.n_field) + 1j * rng.standard_normal(f.n_field)
+    pts = np.ascontiguousarray(surf.edge_centers)
+    Es = np.empty((3, M), dtype=np.complex128)
+    Cs = np.empty((3, M), dtype=np.complex128)
+    own = np.empty(M, dtype=np.int32)
+    samp = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        fctx.call("emg_field_points", ptr(xs), M, ptr(pts), ptr(Es), ptr(Cs), ptr(own))
+        samp.append(time.perf_counter() - t0)
     E = (1 + rng.random((3, M))) * np.exp(2j * np.pi * rng.random((3, M)))
     H = (1 + rng.random((3, M))) * np.exp(2j * np.pi * rng.random((3, M))) / 376.73031366857
     T, P = np.meshgrid(np.radians(np.arange(181.0)), np.radians(np.arange(361.0)), indexing="ij")
@@ -265,7 +283,10 @@ def farfield_bench(n: int = 60, reps: int = 5) -> dict:
                        f"{T.shape[0]} directions (1 degree), synthetic fields",
            "samples": M, "directions": int(T.shape[0]), "ms_per_call": round(dt * 1e3, 3),
            "pairs_per_s": pairs / dt, "flops_per_pair": 53,
-           "note": "wall time per call, host copies included; kernel time in the rocprof summary"}
+           "note": "wall time per call, host copies included; kernel time in the rocprof summary",
+           "field_sampling": {"tets": int(m.n_tets), "points": M, "unowned": int((own < 0).sum()),
+                              "first_call_ms": round(samp[0] * 1e3, 2),
+                              "ms_per_call": round(float(np.mean(samp[1:])) * 1e3, 3)}}
     if not os.environ.get("EMG_NO_CPU_BASELINE"):
         from oracle import farfield as off
         sel = np.arange(0, T.shape[0], 131)
