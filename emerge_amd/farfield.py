@@ -78,3 +78,55 @@ def stratton_chu(Ein, Hin, surface: FarFieldSurface, theta, phi, k0: float, ctx:
     ctx.call("emg_farfield", M, ptr(E), ptr(H), ptr(pos), ptr(wn), N, ptr(th), ptr(ph), float(k0),
              ptr(Eout), ptr(Hout))
     return Eout, Hout, wn
+
+
+class _Mirrored:
+    """Samples and weighted normals of a surface mirrored in the planes
+    x = 0, y = 0 and/or z = 0 (SurfaceMesh.flip, E/mesh3d.py:760-787)."""
+
+    def __init__(self, surface, axes: str):
+        self.edge_centers = surface.edge_centers.copy()
+        self.weighted_normals = surface.weighted_normals.copy()
+        for a in axes:
+            k = "xyz".index(a)
+            self.edge_centers[k] = -self.edge_centers[k]
+            self.weighted_normals[k] = -self.weighted_normals[k]
+
+
+def _signs(syms) -> tuple[np.ndarray, np.ndarray]:
+    """Component signs of E and H for the symmetry planes `syms` ('Ex' = E
+    symmetry in x = 0, 'Hy' = H symmetry in y = 0, ...), as _EHSign
+    (MW/microwave_data.py:253-303): 'E' flips all of E, 'H' all of H, and
+    the axis flips that component of both."""
+    se = np.ones(3)
+    sh = np.ones(3)
+    for s in syms:
+        f, c = s[0], s[1].lower()
+        if f == "E":
+            se = -se
+        elif f == "H":
+            sh = -sh
+        k = "xyz".index(c)
+        se[k] = -se[k]
+        sh[k] = -sh[k]
+    return se, sh
+
+
+def farfield_symmetric(Ein, Hin, surface, theta, phi, k0: float, syms=None, ctx: Context | None = None):
+    """``stratton_chu`` plus the mirror images of the surface for symmetry
+    planes (MW/microwave_data.py:790-861): with planes s1..sn every non-empty
+    combination of them adds the far field of the mirrored surface with the
+    component signs of that combination."""
+    Eff, Hff, _ = stratton_chu(Ein, Hin, surface, theta, phi, k0, ctx=ctx)
+    syms = list(syms or [])
+    if len(syms) > 3:
+        raise ValueError("at most three symmetry planes")
+    from itertools import combinations
+    for r in range(1, len(syms) + 1):
+        for combo in combinations(syms, r):
+            se, sh = _signs(combo)
+            surf = _Mirrored(surface, "".join(s[1].lower() for s in combo))
+            E2, H2, _ = stratton_chu(Ein * se[:, None], Hin * sh[:, None], surf, theta, phi, k0, ctx=ctx)
+            Eff = Eff + E2
+            Hff = Hff + H2
+    return Eff, Hff

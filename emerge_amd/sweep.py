@@ -215,6 +215,22 @@ class FrequencySweep:
         H[:, ok] = C[:, ok] / (-1j * 2 * np.pi * freq * mu0 * mu_r[own[ok]])[None, :]
         return E, H
 
+    def farfield(self, field: np.ndarray, freq: float, tri_ids, theta, phi, origin=None, syms=None, mu_r=None):
+        """Far field of a solution over the boundary triangles `tri_ids`
+        (MWField.farfield, MW/microwave_data.py:790-861): E and H sampled at
+        the surface's edge centres on the device, the Stratton-Chu sum on the
+        device, mirror images for the symmetry planes `syms`.  Returns
+        (Eff, Hff, Ptot) with Ptot = sum_j Re(S_j . wn_j), S = E x conj(H) / 2
+        the time-averaged Poynting vector at the samples."""
+        from .farfield import FarFieldSurface, farfield_symmetric
+        surf = FarFieldSurface(self.mesh, tri_ids, origin=origin)
+        E, H = self.interpolate(field, *surf.exyz, freq=freq, mu_r=mu_r)
+        k0 = 2 * np.pi * freq / C0
+        Eff, Hff = farfield_symmetric(E, H, surf, theta, phi, k0, syms=syms, ctx=self.ctx)
+        S = 0.5 * np.cross(E, np.conj(H), axis=0)
+        Ptot = float(np.sum(S * surf.weighted_normals).real)
+        return Eff, Hff, Ptot
+
     # ------------------------------------------------------------------ setup
     def pec_dofs(self) -> np.ndarray:
         """DOFs removed by PEC faces: both DOFs of every edge of a PEC triangle

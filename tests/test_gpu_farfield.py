@@ -113,3 +113,43 @@ def test_aperture_far_field_from_a_solved_sweep():
     assert scale > 0
     assert np.abs(Eo - Efr).max() <= 1e-10 * scale, np.abs(Eo - Efr).max() / scale
     assert np.abs(Ho - Hfr).max() <= 1e-10 * scale / ff.Z0
+
+
+def test_symmetry_plane_reproduces_the_closed_surface(ctx):
+    """A z-dipole's x = 0 plane is a magnetic wall ('Hx': E odd in x, H
+    even): the half of the cube with x > 0 plus its mirror image gives the
+    dipole's far field as the closed cube does."""
+    from emerge_amd.farfield import farfield_symmetric
+    m, bt, _, _ = cube_surface(6)
+    cx = m.nodes[0, m.tris[:, bt]].mean(axis=0)
+    half = FarFieldSurface(m, bt[cx > 0], origin=(0.0, 0.0, 0.0))
+    E, H = dipole_fields(half.edge_centers)
+    T = np.linspace(0.2, np.pi - 0.2, 9)
+    P = np.full(9, 0.7)
+    Eh, _ = farfield_symmetric(E, H, half, T, P, K, syms=["Hx"], ctx=ctx)
+    tho = np.stack([np.cos(T) * np.cos(P), np.cos(T) * np.sin(P), -np.sin(T)])
+    ref = 1j * ff.Z0 * K * np.sin(T) / (4 * np.pi)
+    err = np.abs((Eh * tho).sum(axis=0) - ref).max() / np.abs(ref).max()
+    assert err < 2e-3, err
+    E1, _ = farfield_symmetric(E, H, half, T, P, K, syms=None, ctx=ctx)
+    assert np.abs((E1 * tho).sum(axis=0) - ref).max() > 0.1 * np.abs(ref).max()   # the half alone is not it
+    Ew, _ = farfield_symmetric(E, H, half, T, P, K, syms=["Ex"], ctx=ctx)     # the wrong wall
+    assert np.abs((Ew * tho).sum(axis=0) - ref).max() > 0.1 * np.abs(ref).max()
+
+
+def test_aperture_power_is_the_transmitted_power():
+    """Ptot of MWField.farfield through the absorbing aperture of the WR-90
+    guide (port 1 fed with 1 W) equals the power not reflected, 1 - |S11|^2,
+    to the discretisation: the time-averaged Poynting vector E x H* / 2."""
+    from emerge_amd.sweep import FrequencySweep
+    from tests import wg_case
+    c = wg_case.abc_case(1)
+    m, f, p = c["mesh"], c["basis"], c["problem"]
+    sw = FrequencySweep(m, f, p.eps_r, p.inv_mu_r, c["prod_bcs"])
+    freq = 10e9
+    res = sw.solve([freq], precision="fp64", tol=1e-10)
+    Eff, Hff, Ptot = sw.farfield(res.fields[(0, 1)], freq, c["abc_tris"], np.array([0.1, 0.5]), np.array([0.0, 0.3]))
+    s11 = abs(res.S[0, 0, 0])
+    print(f"Ptot={Ptot:.4f} 1-|S11|^2={1 - s11 ** 2:.4f}", flush=True)
+    assert abs(Ptot - (1 - s11 ** 2)) < 0.05
+    assert np.all(np.isfinite(Eff)) and np.all(np.isfinite(Hff))
