@@ -130,3 +130,20 @@ def farfield_symmetric(Ein, Hin, surface, theta, phi, k0: float, syms=None, ctx:
             Eff = Eff + E2
             Hff = Hff + H2
     return Eff, Hff
+
+
+def arc_on_plane(ref_dir, normal, angle_range_deg, num_points: int = 100):
+    """(theta, phi) of unit directions along an arc in the plane with normal
+    `normal`, angle 0 along `ref_dir` projected into the plane and 90
+    degrees along normal x ref (arc_on_plane, MW/microwave_data.py:42-91)."""
+    n = np.asarray(normal, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    d = np.asarray(ref_dir, dtype=np.float64)
+    d = d - np.dot(d, n) * n
+    if np.linalg.norm(d) < 1e-12:
+        raise ValueError("Reference direction is parallel to the normal vector.")
+    e1 = d / np.linalg.norm(d)
+    e2 = np.cross(n, e1)
+    a = np.deg2rad(np.linspace(angle_range_deg[0], angle_range_deg[1], num_points))
+    u = np.outer(np.cos(a), e1) + np.outer(np.sin(a), e2)
+    return np.arccos(u[:, 2]), np.arctan2(u[:, 1], u[:, 0])

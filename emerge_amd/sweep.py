@@ -231,6 +231,28 @@ class FrequencySweep:
         Ptot = float(np.sum(S * surf.weighted_normals).real)
         return Eff, Hff, Ptot
 
+    def farfield_2d(self, field: np.ndarray, freq: float, tri_ids, ref_direction, plane_normal,
+                    ang_range=(-180, 180), Npoints: int = 201, origin=None, syms=None, mu_r=None):
+        """Far field along a cut (MWField.farfield_2d, MW/microwave_data.py:
+        730-756): returns (angles in radians, Eff (3, N), Hff (3, N), Ptot)."""
+        from .farfield import arc_on_plane
+        theta, phi = arc_on_plane(ref_direction, plane_normal, ang_range, Npoints)
+        Eff, Hff, Ptot = self.farfield(field, freq, tri_ids, theta, phi, origin=origin, syms=syms, mu_r=mu_r)
+        return np.linspace(*ang_range, Npoints) * np.pi / 180, Eff, Hff, Ptot
+
+    def farfield_3d(self, field: np.ndarray, freq: float, tri_ids, thetas=None, phis=None, origin=None,
+                    syms=None, mu_r=None):
+        """Far field on a theta x phi grid (MWField.farfield_3d,
+        MW/microwave_data.py:758-788; defaults 0..pi in 91 and -pi..pi in
+        181 steps): returns (T, P, Eff (3, *T.shape), Hff, Ptot), T and P from
+        meshgrid(thetas, phis)."""
+        thetas = np.linspace(0, np.pi, 91) if thetas is None else np.asarray(thetas)
+        phis = np.linspace(-np.pi, np.pi, 181) if phis is None else np.asarray(phis)
+        T, P = np.meshgrid(thetas, phis)
+        Eff, Hff, Ptot = self.farfield(field, freq, tri_ids, T.ravel(), P.ravel(), origin=origin, syms=syms,
+                                       mu_r=mu_r)
+        return T, P, Eff.reshape((3,) + T.shape), Hff.reshape((3,) + T.shape), Ptot
+
     # ------------------------------------------------------------------ setup
     def pec_dofs(self) -> np.ndarray:
         """DOFs removed by PEC faces: both DOFs of every edge of a PEC triangle

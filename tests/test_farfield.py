@@ -127,3 +127,19 @@ def test_product_surface_matches_oracle(origin):
     s = FarFieldSurface(m, bt, origin=origin)
     np.testing.assert_array_equal(s.edge_centers, pos)
     np.testing.assert_allclose(s.weighted_normals, wn, rtol=0, atol=1e-15 * np.abs(wn).max())
+
+
+def test_arc_on_plane_directions():
+    """Directions of the cut lie in the plane, start along the projected
+    reference direction and reach normal x reference at 90 degrees."""
+    from emerge_amd.farfield import arc_on_plane
+    n = np.array([0.0, 1.0, 1.0]) / np.sqrt(2)
+    t, p = arc_on_plane((1.0, 0.3, 0.0), (0.0, 1.0, 1.0), (-180, 180), 9)
+    u = np.stack([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)])
+    assert np.abs(n @ u).max() < 1e-12
+    d = np.array([1.0, 0.3, 0.0]) - np.dot([1.0, 0.3, 0.0], n) * n
+    e1 = d / np.linalg.norm(d)
+    np.testing.assert_allclose(u[:, 4], e1, atol=1e-12)           # angle 0
+    np.testing.assert_allclose(u[:, 6], np.cross(n, e1), atol=1e-12)   # +90 degrees
+    with pytest.raises(ValueError):
+        arc_on_plane((0.0, 1.0, 1.0), (0.0, 1.0, 1.0), (0, 90), 3)

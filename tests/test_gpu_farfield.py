@@ -153,3 +153,23 @@ def test_aperture_power_is_the_transmitted_power():
     print(f"Ptot={Ptot:.4f} 1-|S11|^2={1 - s11 ** 2:.4f}", flush=True)
     assert abs(Ptot - (1 - s11 ** 2)) < 0.05
     assert np.all(np.isfinite(Eff)) and np.all(np.isfinite(Hff))
+
+
+def test_cut_and_grid_wrappers_agree_with_farfield():
+    from emerge_amd.farfield import arc_on_plane
+    from emerge_amd.sweep import FrequencySweep
+    from tests import wg_case
+    c = wg_case.abc_case(1)
+    m, f, p = c["mesh"], c["basis"], c["problem"]
+    sw = FrequencySweep(m, f, p.eps_r, p.inv_mu_r, c["prod_bcs"])
+    freq = 10e9
+    x = sw.solve([freq], precision="fp64", tol=1e-10).fields[(0, 1)]
+    ang, E2, H2, P2 = sw.farfield_2d(x, freq, c["abc_tris"], (0, 0, 1), (0, 1, 0), (-90, 90), 7)
+    t, ph = arc_on_plane((0, 0, 1), (0, 1, 0), (-90, 90), 7)
+    E1, H1, P1 = sw.farfield(x, freq, c["abc_tris"], t, ph)
+    np.testing.assert_allclose(E2, E1, rtol=0, atol=1e-13 * np.abs(E1).max())
+    assert ang.shape == (7,) and P2 == P1
+    T, P, E3, H3, P3 = sw.farfield_3d(x, freq, c["abc_tris"], np.linspace(0, 1, 4), np.linspace(0, 2, 5))
+    assert E3.shape == (3, 5, 4) and T.shape == (5, 4)
+    E4, _, _ = sw.farfield(x, freq, c["abc_tris"], T.ravel(), P.ravel())
+    np.testing.assert_allclose(E3.reshape(3, -1), E4, rtol=0, atol=1e-13 * np.abs(E4).max())
