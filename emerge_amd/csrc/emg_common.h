@@ -147,6 +147,12 @@ struct emg_ctx {
     emg::DBuf<int64_t> grid_ptr;
     emg::DBuf<int32_t> grid_tets;
 
+    // buffers of emg_field_points and emg_farfield, kept between calls (a
+    // free would synchronise the device)
+    emg::DBuf<double> fp_x, fp_pts, fp_E, fp_C;
+    emg::DBuf<int32_t> fp_own;
+    emg::DBuf<double> ff_E, ff_H, ff_pos, ff_wn, ff_th, ff_ph, ff_emag, ff_emax, ff_rec, ff_part, ff_Eo, ff_Ho;
+
     // Krylov method of the solves (emg_solver_config)
     int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part
     int idr_s = 4;

@@ -188,7 +188,18 @@ extern "C" int emg_farfield(emg_ctx* c, int64_t M, const double* E, const double
     if (M > (int64_t(1) << 31) || N > (int64_t(1) << 31))
         return fail(c, EMG_ERR_ARG, "emg_farfield: more than 2^31 samples or directions");
     hipStream_t st = c->stream;
-    DBuf<double> dE, dH, dpos, dwn, dth, dph, emag, emax, rec, part, dEo, dHo;
+    DBuf<double>& dE = c->ff_E;
+    DBuf<double>& dH = c->ff_H;
+    DBuf<double>& dpos = c->ff_pos;
+    DBuf<double>& dwn = c->ff_wn;
+    DBuf<double>& dth = c->ff_th;
+    DBuf<double>& dph = c->ff_ph;
+    DBuf<double>& emag = c->ff_emag;
+    DBuf<double>& emax = c->ff_emax;
+    DBuf<double>& rec = c->ff_rec;
+    DBuf<double>& part = c->ff_part;
+    DBuf<double>& dEo = c->ff_Eo;
+    DBuf<double>& dHo = c->ff_Ho;
     EMG_HIP(c, dE.alloc(6 * M));
     EMG_HIP(c, dH.alloc(6 * M));
     EMG_HIP(c, dpos.alloc(3 * M));

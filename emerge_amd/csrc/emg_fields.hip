@@ -362,8 +362,11 @@ extern "C" int emg_field_points(emg_ctx* c, const double* field, int64_t P, cons
     g.h = c->grid_h;
     hipStream_t st = c->stream;
     const int64_t N = c->n_field;
-    DBuf<double> dx, dp, dE, dC;
-    DBuf<int32_t> dow;
+    DBuf<double>& dx = c->fp_x;
+    DBuf<double>& dp = c->fp_pts;
+    DBuf<double>& dE = c->fp_E;
+    DBuf<double>& dC = c->fp_C;
+    DBuf<int32_t>& dow = c->fp_own;
     EMG_HIP(c, dx.alloc(2 * N));
     EMG_HIP(c, dp.alloc(3 * P));
     EMG_HIP(c, dE.alloc(6 * P));

@@ -345,3 +345,16 @@ def test_mixed_safe32_modes_match_superlu(sweep, safe32, nf):
         o = wg_case.oracle_solution(float(f))
         assert np.abs(res.S[i] - o["S"]).max() <= 1e-6
     print(f"safe32={safe32} iters={res.iters.tolist()} escalated={res.fp64_escalated}")
+
+
+def test_auto_precision_policy(sweep):
+    """precision='auto': complex64 inner solves on the coarse guide at
+    X band, FP64 where the mesh is fine against the wavelength."""
+    assert sweep.choose_precision([8e9, 12e9]) == "mixed"
+    assert sweep.choose_precision([1e9]) == "fp64"
+    o = wg_case.oracle_solution(F0)
+    res = sweep.solve([F0], precision="auto", tol=1e-8)
+    assert res.precision == "mixed"
+    assert np.abs(res.S[0] - o["S"]).max() <= 1e-6
+    with pytest.raises(ValueError):
+        sweep.solve([F0], precision="fp16")
