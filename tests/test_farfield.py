@@ -143,3 +143,18 @@ def test_arc_on_plane_directions():
     np.testing.assert_allclose(u[:, 6], np.cross(n, e1), atol=1e-12)   # +90 degrees
     with pytest.raises(ValueError):
         arc_on_plane((0.0, 1.0, 1.0), (0.0, 1.0, 1.0), (0, 90), 3)
+
+
+@pytest.mark.parametrize("syms,se,sh", [
+    (["Ex"], (1, -1, -1), (-1, 1, 1)),       # E wall x = 0: tangential E odd
+    (["Hx"], (-1, 1, 1), (1, -1, -1)),       # H wall x = 0
+    (["Ex", "Hy"], (1, 1, -1), (1, 1, -1)),
+    (["Hz", "Ez"], (1, 1, 1), (1, 1, 1)),    # two flips of every component
+])
+def test_symmetry_signs(syms, se, sh):
+    """Component signs of the mirrored fields, as _EHSign applies them
+    (MW/microwave_data.py:253-303)."""
+    from emerge_amd.farfield import _signs
+    a, b = _signs(syms)
+    np.testing.assert_array_equal(a, se)
+    np.testing.assert_array_equal(b, sh)
