@@ -149,7 +149,7 @@ def test_arc_on_plane_directions():
     (["Ex"], (1, -1, -1), (-1, 1, 1)),       # E wall x = 0: tangential E odd
     (["Hx"], (-1, 1, 1), (1, -1, -1)),       # H wall x = 0
     (["Ex", "Hy"], (1, 1, -1), (1, 1, -1)),
-    (["Hz", "Ez"], (1, 1, 1), (1, 1, 1)),    # two flips of every component
+    (["Hz", "Ez"], (-1, -1, -1), (-1, -1, -1)),   # z flipped twice, then E and H once each
 ])
 def test_symmetry_signs(syms, se, sh):
     """Component signs of the mirrored fields, as _EHSign applies them
