@@ -39,6 +39,8 @@ SIGNATURES: dict[str, tuple] = {
     "emg_values_get": (_c_int, [_p, _p, _p]),
     "emg_system_reduce": (_c_int, [_p, _i64, _p]),
     "emg_system_refresh": (_c_int, [_p]),
+    "emg_dof_maps": (_c_int, [_p, _p, _p, _p]),
+    "emg_pec_keep": (_c_int, [_p, _i64, _p, _p, _p]),
     "emg_field_points": (_c_int, [_p, _p, _i64, _p, _p, _p, _p]),
     "emg_farfield": (_c_int, [_p, _i64, _p, _p, _p, _p, _i64, _p, _p, ctypes.c_double, _p, _p]),
     "emg_system_pattern_get": (_c_int, [_p, _p, _p, _pi64]),

@@ -47,6 +47,25 @@ class Nedelec2:
         r2f[7] = tr + 2 * ne + nf
         self.tri_to_field = r2f
 
+    @classmethod
+    def on_device(cls, mesh: Mesh3D, ctx) -> "Nedelec2":
+        """The same DOF maps built on the GPU (``emg_dof_maps``) from the
+        topology that ``Mesh3D.on_device(nodes, tets, ctx)`` left in `ctx`."""
+        from ._lib import ptr
+        obj = cls.__new__(cls)
+        obj.mesh = mesh
+        ne, nf, T = mesh.n_edges, mesh.n_tris, mesh.n_tets
+        obj.n_edges, obj.n_tris, obj.n_tets = ne, nf, T
+        obj.n_field = 2 * ne + 2 * nf
+        t2f = np.empty((20, T), dtype=np.int32)
+        r2f = np.empty((8, nf), dtype=np.int32)
+        e2f = np.empty((2, ne), dtype=np.int32)
+        ctx.call("emg_dof_maps", ptr(t2f), ptr(r2f), ptr(e2f))
+        obj.tet_to_field = t2f.astype(np.int64)
+        obj.tri_to_field = r2f.astype(np.int64)
+        obj.edge_to_field = e2f.astype(np.int64)
+        return obj
+
     def tri_rowcol(self, tri_ids: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         """Row/col index of each 8x8 entry of the given triangles, row-major
         per triangle (femdata.py:105-121)."""
@@ -54,3 +73,18 @@ class Nedelec2:
         rows = np.repeat(idx.T, 8, axis=1).reshape(-1)
         cols = np.tile(idx.T, (1, 8)).reshape(-1)
         return rows, cols
+
+
+def pec_keep_ids(ctx, pec_tri_ids: np.ndarray) -> np.ndarray:
+    """Ascending DOFs kept by the PEC elimination (ASM/assembler.py:274-307,
+    406-407) on the topology in `ctx` (``emg_pec_keep``): every DOF except
+    both DOFs of each edge of the PEC triangles and their 8 face DOFs."""
+    import ctypes
+
+    from ._lib import ptr
+    tri = np.ascontiguousarray(pec_tri_ids, dtype=np.int32)
+    n = ctypes.c_int64()
+    ctx.call("emg_pec_keep", tri.shape[0], ptr(tri) if tri.size else None, ctypes.byref(n), None)
+    out = np.empty(n.value, dtype=np.int32)
+    ctx.call("emg_pec_keep", tri.shape[0], ptr(tri) if tri.size else None, ctypes.byref(n), ptr(out))
+    return out.astype(np.int64)

@@ -199,6 +199,63 @@ int unique_keys(emg_ctx* c, int64_t n, DBuf<uint64_t>& in, DBuf<uint64_t>& out, 
     return EMG_OK;
 }
 
+// DOF maps of the second-order Nedelec space from the device topology
+// (Nedelec2.__init__, E/elements/nedelec2.py:42-58): blocks
+// [edge 1st | face 1st | edge 2nd | face 2nd] of sizes nE, nF, nE, nF.
+__global__ void k_dof_maps(int64_t T, int64_t nE, int64_t nF, const int32_t* __restrict__ t2e,
+                           const int32_t* __restrict__ t2t, const int32_t* __restrict__ r2e,
+                           int32_t* __restrict__ t2f, int32_t* __restrict__ r2f, int32_t* __restrict__ e2f) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int32_t oF = int32_t(nE), oE2 = int32_t(nE + nF), oF2 = int32_t(2 * nE + nF);
+    if (i < T) {
+        for (int k = 0; k < 6; ++k) {
+            const int32_t e = t2e[k * T + i];
+            t2f[k * T + i] = e;
+            t2f[(10 + k) * T + i] = e + oE2;
+        }
+        for (int k = 0; k < 4; ++k) {
+            const int32_t f = t2t[k * T + i];
+            t2f[(6 + k) * T + i] = f + oF;
+            t2f[(16 + k) * T + i] = f + oF2;
+        }
+    }
+    if (i < nF) {
+        for (int k = 0; k < 3; ++k) {
+            const int32_t e = r2e[k * nF + i];
+            r2f[k * nF + i] = e;
+            r2f[(4 + k) * nF + i] = e + oE2;
+        }
+        r2f[3 * nF + i] = int32_t(i) + oF;
+        r2f[7 * nF + i] = int32_t(i) + oF2;
+    }
+    if (i < nE) {
+        e2f[i] = int32_t(i);
+        e2f[nE + i] = int32_t(i) + oE2;
+    }
+}
+
+// removed[d] = 1 for both DOFs of every edge of a PEC triangle and all 8 DOFs
+// of the triangle (ASM/assembler.py:290-307); every writer stores the same 1
+__global__ void k_pec_mark(int64_t n, const int32_t* __restrict__ tri, int64_t nE, int64_t nF,
+                           const int32_t* __restrict__ r2e, int32_t* __restrict__ keep) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t t = tri[k];
+    for (int j = 0; j < 3; ++j) {
+        const int64_t e = r2e[j * nF + t];
+        keep[e] = 0;
+        keep[e + nE + nF] = 0;
+    }
+    keep[t + nE] = 0;
+    keep[t + 2 * nE + nF] = 0;
+}
+
+__global__ void k_keep_scatter(int64_t N, const int32_t* __restrict__ keep, const int32_t* __restrict__ pos,
+                               int32_t* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < N && keep[i]) out[pos[i]] = int32_t(i);
+}
+
 }  // namespace
 
 extern "C" int emg_topology_build(emg_ctx* c, int64_t V, const double* xyz_soa, int64_t T, const int32_t* tets,
@@ -282,5 +339,71 @@ extern "C" int emg_topology_get(emg_ctx* c, int32_t* edges, int32_t* tris, int32
     EMG_HIP(c, get(tri_to_tet, c->topo_r2t.p, sizeof(int32_t) * 2 * nF));
     EMG_HIP(c, get(edge_lengths, c->topo_len.p, sizeof(double) * nE));
     EMG_HIP(c, hipStreamSynchronize(s));
+    return EMG_OK;
+}
+
+extern "C" int emg_dof_maps(emg_ctx* c, int32_t* tet_to_field, int32_t* tri_to_field, int32_t* edge_to_field) {
+    EMG_CHECK_CTX(c);
+    if (c->topo_T == 0) return fail(c, EMG_ERR_STATE, "emg_dof_maps: no topology built");
+    const int64_t T = c->topo_T, nE = c->topo_nE, nF = c->topo_nF;
+    if (2 * (nE + nF) > INT_MAX) return fail(c, EMG_ERR_ARG, "emg_dof_maps: more than 2^31 DOFs");
+    hipStream_t s = c->stream;
+    DBuf<int32_t> t2f, r2f, e2f;
+    EMG_HIP(c, t2f.alloc(20 * T));
+    EMG_HIP(c, r2f.alloc(8 * nF));
+    EMG_HIP(c, e2f.alloc(2 * nE));
+    const int64_t n = std::max(T, std::max(nE, nF));
+    k_dof_maps<<<unsigned((n + 255) / 256), 256, 0, s>>>(T, nE, nF, c->topo_t2e.p, c->topo_t2t.p, c->topo_r2e.p,
+                                                         t2f.p, r2f.p, e2f.p);
+    EMG_HIP(c, hipGetLastError());
+    if (tet_to_field)
+        EMG_HIP(c, hipMemcpyAsync(tet_to_field, t2f.p, sizeof(int32_t) * 20 * T, hipMemcpyDeviceToHost, s));
+    if (tri_to_field)
+        EMG_HIP(c, hipMemcpyAsync(tri_to_field, r2f.p, sizeof(int32_t) * 8 * nF, hipMemcpyDeviceToHost, s));
+    if (edge_to_field)
+        EMG_HIP(c, hipMemcpyAsync(edge_to_field, e2f.p, sizeof(int32_t) * 2 * nE, hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipStreamSynchronize(s));
+    return EMG_OK;
+}
+
+extern "C" int emg_pec_keep(emg_ctx* c, int64_t n_pec, const int32_t* pec_tris, int64_t* n_keep, int32_t* keep_ids) {
+    EMG_CHECK_CTX(c);
+    if (c->topo_T == 0) return fail(c, EMG_ERR_STATE, "emg_pec_keep: no topology built");
+    if (n_pec < 0 || (n_pec > 0 && !pec_tris) || !n_keep) return fail(c, EMG_ERR_ARG, "emg_pec_keep: bad arguments");
+    const int64_t nE = c->topo_nE, nF = c->topo_nF, N = 2 * (nE + nF);
+    if (N > INT_MAX) return fail(c, EMG_ERR_ARG, "emg_pec_keep: more than 2^31 DOFs");
+    for (int64_t k = 0; k < n_pec; ++k)
+        if (pec_tris[k] < 0 || pec_tris[k] >= nF)
+            return fail(c, EMG_ERR_ARG, "emg_pec_keep: triangle id %lld out of range", (long long)pec_tris[k]);
+    hipStream_t s = c->stream;
+    DBuf<int32_t> keep, pos, tri, out;
+    EMG_HIP(c, keep.alloc(N));
+    EMG_HIP(c, pos.alloc(N));
+    const unsigned bn = unsigned((N + 255) / 256);
+    k_fill_i32<<<bn, 256, 0, s>>>(N, keep.p, 1);
+    if (n_pec > 0) {
+        EMG_HIP(c, tri.alloc(n_pec));
+        EMG_HIP(c, hipMemcpyAsync(tri.p, pec_tris, sizeof(int32_t) * n_pec, hipMemcpyHostToDevice, s));
+        k_pec_mark<<<unsigned((n_pec + 255) / 256), 256, 0, s>>>(n_pec, tri.p, nE, nF, c->topo_r2e.p, keep.p);
+    }
+    EMG_HIP(c, hipGetLastError());
+    size_t sb = 0;
+    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, keep.p, pos.p, int(N), s));
+    DBuf<unsigned char> tmp;
+    EMG_HIP(c, tmp.alloc(std::max<size_t>(sb, 1)));
+    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp.p, sb, keep.p, pos.p, int(N), s));
+    int32_t last_pos = 0, last_keep = 0;
+    EMG_HIP(c, hipMemcpyAsync(&last_pos, pos.p + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipMemcpyAsync(&last_keep, keep.p + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipStreamSynchronize(s));
+    const int64_t nk = int64_t(last_pos) + last_keep;
+    *n_keep = nk;
+    if (keep_ids && nk > 0) {
+        EMG_HIP(c, out.alloc(nk));
+        k_keep_scatter<<<bn, 256, 0, s>>>(N, keep.p, pos.p, out.p);
+        EMG_HIP(c, hipGetLastError());
+        EMG_HIP(c, hipMemcpyAsync(keep_ids, out.p, sizeof(int32_t) * nk, hipMemcpyDeviceToHost, s));
+        EMG_HIP(c, hipStreamSynchronize(s));
+    }
     return EMG_OK;
 }

@@ -258,6 +258,19 @@ int emg_topology_build(emg_ctx* ctx, int64_t V, const double* xyz_soa, int64_t T
 int emg_topology_get(emg_ctx* ctx, int32_t* edges, int32_t* tris, int32_t* tet_to_edge, int32_t* tet_to_tri,
                      int32_t* tri_to_edge, int32_t* tri_to_tet, double* edge_lengths);
 
+/* DOF maps of the second-order Nedelec space from the topology of the last
+ * emg_topology_build (Nedelec2.__init__, E/elements/nedelec2.py:42-58):
+ * tet_to_field [20][T], tri_to_field [8][nF], edge_to_field [2][nE], int32
+ * row-major, blocks [edge 1st | face 1st | edge 2nd | face 2nd].  Any
+ * output may be NULL. */
+int emg_dof_maps(emg_ctx* ctx, int32_t* tet_to_field, int32_t* tri_to_field, int32_t* edge_to_field);
+/* DOFs kept by the PEC elimination (ASM/assembler.py:274-307, 406-407) on
+ * that topology: every DOF except both DOFs of each edge of the n_pec PEC
+ * triangles and the 8 DOFs of each of those triangles, ascending.  *n_keep
+ * receives the count; keep_ids (capacity 2 (nE + nF)) may be NULL to ask
+ * for the count only. */
+int emg_pec_keep(emg_ctx* ctx, int64_t n_pec, const int32_t* pec_tris, int64_t* n_keep, int32_t* keep_ids);
+
 /* Auxiliary gradient space of the COCG preconditioner (method 2 of
  * emg_sweep_solve): the inner solves apply z = D^-1 r + G D_g^-1 G^T r with
  * D the diagonal of the operator, G (n_red x n_nodal, real CSR g_*, its

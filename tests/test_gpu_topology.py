@@ -47,3 +47,28 @@ def test_device_topology_is_bit_exact(hip_ctx, name, mesh):
     e = dev.get_edge(int(dev.edges[0, 5]), int(dev.edges[1, 5]))
     assert e == 5
     assert np.array_equal(dev.boundary_tris(), host.boundary_tris())
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.2])
+def test_dof_maps_and_pec_keep_match_host(jitter):
+    """emg_dof_maps and emg_pec_keep on the device topology equal
+    Nedelec2's maps and the complement of the PEC DOFs, bit for bit."""
+    from emerge_amd._lib import Context
+    from emerge_amd.basis import Nedelec2, pec_keep_ids
+    from emerge_amd.mesh import Mesh3D, kuhn_box
+    nodes, tets = kuhn_box(7, 4, 9, size=(0.02286, 0.01016, 0.04), jitter=jitter, seed=4)
+    ctx = Context(0)
+    md = Mesh3D.on_device(nodes, tets, ctx)
+    fd = Nedelec2.on_device(md, ctx)
+    mh = Mesh3D(nodes, tets)
+    fh = Nedelec2(mh)
+    for name in ("tet_to_field", "tri_to_field", "edge_to_field"):
+        np.testing.assert_array_equal(getattr(fd, name), getattr(fh, name))
+    assert fd.n_field == fh.n_field
+    bt = mh.boundary_tris()
+    pec = np.setdiff1d(bt, np.concatenate([mh.plane_tris(2, 0.0), mh.plane_tris(2, 0.04)]))
+    removed = np.unique(np.concatenate([fh.edge_to_field[:, mh.tri_to_edge[:, pec].reshape(-1)].reshape(-1),
+                                        fh.tri_to_field[:, pec].reshape(-1)]))
+    keep = np.setdiff1d(np.arange(fh.n_field), removed)
+    np.testing.assert_array_equal(pec_keep_ids(ctx, pec), keep)
+    np.testing.assert_array_equal(pec_keep_ids(ctx, np.zeros(0, dtype=np.int64)), np.arange(fh.n_field))
