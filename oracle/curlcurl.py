@@ -122,12 +122,18 @@ def _cross(a, b):
     return np.stack([a[1]*b[2] - a[2]*b[1], a[2]*b[0] - a[0]*b[2], a[0]*b[1] - a[1]*b[0]])
 
 
-def ned2_tet_stiff_mass(verts, edge_lengths, lem, ltm, Ms, Mm):
+def ned2_tet_stiff_mass(verts, edge_lengths, lem, ltm, Ms, Mm, exact_face_mass=False):
     """ASM/curlcurl.py:180-412 vectorised over T tets.
 
     verts (3,4,T) f64; edge_lengths (6,T); lem (2,6,T) / ltm (3,4,T) local
     edge/face maps; Ms = inv(mu_r) and Mm = eps_r as (3,3,T) complex.
     Returns Dmat (curl-curl) and Fmat (mass), each (20,20,T) complex128.
+
+    ``exact_face_mass=True`` is for pins only: it replaces the reference's
+    face-face monomial lam_B lam_C lam_D lam_F (ASM/curlcurl.py:361) by the
+    integral's lam_B lam_C lam_D lam_E, which makes the mass matrix the
+    exact Gram matrix of the basis (symmetric); the default reproduces the
+    reference.
     """
     T = verts.shape[2]
     Dm = np.zeros((20, 20, T), dtype=np.complex128)
@@ -255,7 +261,8 @@ def ned2_tet_stiff_mass(verts, edge_lengths, lem, ltm, Ms, Mm):
             Dm[ei + 16, ej + 16] = Lab1*Lab2*(4*vef*ab_cd + 2*vde*ab_cf - 2*vce*ab_df
                                               + 2*vbf*ae_cd + vbd*ae_cf - vbc*ae_df + l12)
             v_abcd, v_abdf = VC(A, B, C, D), VC(A, B, D, F)
-            v_bcde, v_bdef = VC(B, C, D, F), VC(B, E, D, F)
+            v_bcde = VC(B, C, D, E) if exact_face_mass else VC(B, C, D, F)
+            v_bdef = VC(B, E, D, F)
             v_abcf, v_bcef = VC(A, B, F, C), VC(B, E, F, C)
             v_acde, v_adef = VC(E, A, C, D), VC(E, A, D, F)
             v_acef = VC(A, C, E, F)
@@ -277,7 +284,7 @@ def local_maps(tets, edges, tris, tet_to_field, nedges):
 
 
 def matrix_builder(nodes, tets, tris, edges, all_edge_lengths, tet_to_field, tet_to_edge,
-                   ur, er, chunk: int = 20000):
+                   ur, er, chunk: int = 20000, exact_face_mass: bool = False):
     """ASM/curlcurl.py:419-466: COO data for E (curl-curl) and B (mass).
 
     Entry (i, j) of tet t sits at t*400 + 20*i + j with row tet_to_field[i, t]
@@ -296,7 +303,8 @@ def matrix_builder(nodes, tets, tris, edges, all_edge_lengths, tet_to_field, tet
         lem, ltm = local_maps(tets[:, sl], edges, tris, tet_to_field[:, sl], nedges)
         el = all_edge_lengths[tet_to_edge[:, sl]]
         verts = nodes[:, tets[:, sl]]  # (3,4,t)
-        Dm, Fm = ned2_tet_stiff_mass(verts, el, lem, ltm, matinv(ur[:, :, sl]), er[:, :, sl])
+        Dm, Fm = ned2_tet_stiff_mass(verts, el, lem, ltm, matinv(ur[:, :, sl]), er[:, :, sl],
+                                     exact_face_mass=exact_face_mass)
         idx = tet_to_field[:, sl]  # (20,t)
         n = e - s
         r = np.broadcast_to(idx.T[:, :, None], (n, 20, 20)).reshape(-1)
@@ -308,13 +316,14 @@ def matrix_builder(nodes, tets, tris, edges, all_edge_lengths, tet_to_field, tet
     return dataE, dataB, rows, cols
 
 
-def tet_mass_stiffness_matrices(mesh, tet_to_field, er, ur):
+def tet_mass_stiffness_matrices(mesh, tet_to_field, er, ur, exact_face_mass: bool = False):
     """ASM/curlcurl.py:145-173: global E, B as scipy CSR (COO->CSR sums
-    duplicates, keeps explicit zeros, sorts columns, int32 indices)."""
+    duplicates, keeps explicit zeros, sorts columns, int32 indices).
+    ``exact_face_mass`` as in ``ned2_tet_stiff_mass`` (pins only)."""
     N = 2 * mesh.n_edges + 2 * mesh.n_tris
     dataE, dataB, rows, cols = matrix_builder(mesh.nodes, mesh.tets, mesh.tris, mesh.edges,
                                               mesh.edge_lengths, tet_to_field,
-                                              mesh.tet_to_edge, ur, er)
+                                              mesh.tet_to_edge, ur, er, exact_face_mass=exact_face_mass)
     E = coo_matrix((dataE, (rows, cols)), shape=(N, N)).tocsr()
     B = coo_matrix((dataB, (rows, cols)), shape=(N, N)).tocsr()
     return E, B

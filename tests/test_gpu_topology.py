@@ -72,3 +72,35 @@ def test_dof_maps_and_pec_keep_match_host(jitter):
     keep = np.setdiff1d(np.arange(fh.n_field), removed)
     np.testing.assert_array_equal(pec_keep_ids(ctx, pec), keep)
     np.testing.assert_array_equal(pec_keep_ids(ctx, np.zeros(0, dtype=np.int64)), np.arange(fh.n_field))
+
+
+@pytest.mark.parametrize("name,mesh", list(_cases()), ids=lambda v: v if isinstance(v, str) else "")
+def test_device_topology_and_dof_maps_match_oracle(hip_ctx, name, mesh):
+    """Device arrays against the set/dict oracle (oracle.topology, restating
+    E/mesh3d.py:386-498 and E/elements/nedelec2.py:42-58) directly, and the
+    device PEC solve_ids against the oracle's elimination
+    (ASM/assembler.py:274-307, 406-407 as oracle.fem.assemble_freq_matrix)."""
+    from emerge_amd.basis import Nedelec2, pec_keep_ids
+    from oracle import topology as otopo
+    nodes, tets = mesh
+    print(f"topology vs oracle: {name} T={tets.shape[1]}", flush=True)
+    ref = otopo.pre_update(nodes, tets)
+    dev = Mesh3D.on_device(nodes, tets, hip_ctx)
+    for f in FIELDS:
+        np.testing.assert_array_equal(getattr(dev, f), ref[f], err_msg=f"{name} {f}")
+    fd = Nedelec2.on_device(dev, hip_ctx)
+    maps = otopo.nedelec2_maps(ref, tets.shape[1])
+    for key in ("tet_to_field", "tri_to_field", "edge_to_field"):
+        np.testing.assert_array_equal(getattr(fd, key), maps[key], err_msg=f"{name} {key}")
+    assert fd.n_field == maps["n_field"]
+    # PEC on the boundary triangles whose centroid lies in the lower half
+    bt = dev.boundary_tris()
+    zc = nodes[2][dev.tris[:, bt]].mean(axis=0)
+    pec = bt[zc < np.median(zc)]
+    pec_ids = set()
+    for e in ref["tri_to_edge"][:, pec].ravel():
+        pec_ids.update(int(i) for i in maps["edge_to_field"][:, e])
+    for t in pec:
+        pec_ids.update(int(i) for i in maps["tri_to_field"][:, t])
+    solve_ids = np.array([i for i in range(maps["n_field"]) if i not in pec_ids])
+    np.testing.assert_array_equal(pec_keep_ids(hip_ctx, pec), solve_ids)
