@@ -1,12 +1,32 @@
 // Robin / port surface terms and the PEC-reduced system.
 //
-// Per boundary triangle the kernel evaluates the 8x8 matrix of
-// ned2_tri_stiff / ned2_tri_stiff_force without its gamma factor
-// (ASM/robinbc.py:314-427, 126-292) and, for excited ports, the forcing
-// vector b_i = int N_i^t . U_inc dS with the 6-point Dunavant rule
-// (gaus_quad_tri(4), E/mth/optimized.py:31-32, 87-118).  The per-triangle
-// values are then summed into the reduced CSR pattern / reduced DOF vector
-// over host-ordered contribution lists (deterministic segmented sums).
+// On a boundary triangle the second-order Nedelec tet leaves 8 tangential
+// trace functions (vertices 0, 1, 2 ordered by global id; DOF order of
+// tri_to_field: edges (0,1), (1,2), (0,2) of the first kind, face of the
+// first kind, the same edges of the second kind, face of the second kind).
+// With the Whitney form W_pq = lam_p grad lam_q - lam_q grad lam_p they are
+//   edge (p, q), 1st kind:  T = -L_pq lam_p W_pq,   2nd kind: T = -L_pq lam_q W_pq,
+//   face, 1st kind:  T =  L_02 lam_1 W_02,   2nd kind:  T = -L_01 lam_2 W_01,
+// the traces of the volume basis that the reference integrates in
+// ned2_tri_stiff / ned2_tri_stiff_force (ASM/robinbc.py:126-427).  Each
+// T_i is L_i times two monomials lam_a lam_b grad lam_g, so the 8x8 matrix
+// int T_i . T_j dS (without its gamma factor) is a fixed linear form of the
+// six products grad lam_a . grad lam_b:
+//   S[i][j] = L_i L_j (Area / 360) sum_k TS[i][j][k] d_k,
+// with integer TS built at compile time below from the monomial integral
+// int lam^k dS = 2 Area prod(k_i!) / (sum k + 2)! (the reference's
+// AREA_COEFF, ASM/robinbc.py:36-54).  The forcing of excited ports is
+// b_i = -int T_i . U_inc dS on the 6-point Dunavant rule (gaus_quad_tri(4),
+// E/mth/optimized.py:31-32, 87-118); the sign is the reference's signA,
+// which is -1 on every triangle because its local frame is built from the
+// triangle's own edges (ASM/robinbc.py:135-170).  The gradients come from
+// the triangle's edge vectors E1 = P1 - P0, E2 = P2 - P0 and their Gram
+// matrix (no local frame): grad lam_1 = (g22 E1 - g12 E2) / det,
+// grad lam_2 = (g11 E2 - g12 E1) / det, grad lam_0 = -grad lam_1 - grad lam_2,
+// det = g11 g22 - g12^2 = (2 Area)^2.
+// The per-triangle values are then summed into the reduced CSR pattern /
+// reduced DOF vector over host-ordered contribution lists (deterministic
+// segmented sums).
 #include <algorithm>
 
 #include "emg_common.h"
@@ -15,17 +35,83 @@ namespace {
 
 using namespace emg;
 
-// area coefficient 2 prod(k!)/(sum k + 2)! for 4 letters over 3 triangle
-// vertices (ASM/robinbc.py:36-45); letters 1..3, 0 = none (never used here)
-__device__ __forceinline__ double area_coeff4(int a, int b, int c, int d) {
-    int m[4] = {0, 0, 0, 0};
-    m[a]++; m[b]++; m[c]++; m[d]++;
-    const int f[5] = {1, 1, 2, 6, 24};
-    const double num = double(f[m[1]] * f[m[2]] * f[m[3]]);
-    return 2.0 * num / 720.0;  // (4 + 2)! = 720
+namespace trace {
+
+// one term c * lam_a lam_b grad lam_g
+struct Term {
+    int c, a, b, g;
+};
+// trace function: length |P_la - P_lb| times two terms
+struct Fn {
+    int la, lb;
+    Term t0, t1;
+};
+
+constexpr int kTriEdge[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+
+__host__ __device__ constexpr Fn fn(int i) {
+    if (i < 3 || (i >= 4 && i < 7)) {
+        const int e = i < 3 ? i : i - 4;
+        const int p = kTriEdge[e][0], q = kTriEdge[e][1];
+        // -lam_s W_pq = -lam_s lam_p grad lam_q + lam_s lam_q grad lam_p, s = p or q
+        const int s = i < 3 ? p : q;
+        return Fn{p, q, Term{-1, s, p, q}, Term{+1, s, q, p}};
+    }
+    if (i == 3) return Fn{0, 2, Term{+1, 1, 0, 2}, Term{-1, 1, 2, 0}};   //  lam_1 W_02
+    return Fn{0, 1, Term{-1, 2, 0, 1}, Term{+1, 2, 1, 0}};                // -lam_2 W_01
 }
 
+__host__ __device__ constexpr int fact(int k) { return k <= 1 ? 1 : k * fact(k - 1); }
+
+// prod of (multiplicity)! for a degree-4 monomial over the 3 vertices
+__host__ __device__ constexpr int mono(int a, int b, int c, int d) {
+    int m[3] = {0, 0, 0};
+    m[a]++; m[b]++; m[c]++; m[d]++;
+    return fact(m[0]) * fact(m[1]) * fact(m[2]);
+}
+
+// index of grad lam_a . grad lam_b among (00, 01, 02, 11, 12, 22)
+__host__ __device__ constexpr int dot_index(int a, int b) {
+    return a > b ? dot_index(b, a) : (a == 0 ? b : (a == 1 ? 2 + b : 5));
+}
+
+__host__ __device__ constexpr int pair(const Term& x, const Term& y, int k) {
+    return dot_index(x.g, y.g) == k ? x.c * y.c * mono(x.a, x.b, y.a, y.b) : 0;
+}
+
+// TS[i][j][k] (numerator over 360)
+__host__ __device__ constexpr int ts(int i, int j, int k) {
+    const Fn A = fn(i), B = fn(j);
+    return pair(A.t0, B.t0, k) + pair(A.t0, B.t1, k) + pair(A.t1, B.t0, k) + pair(A.t1, B.t1, k);
+}
+
+struct Table {
+    int v[8][8][6];
+    __host__ __device__ constexpr Table() : v{} {
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j)
+                for (int k = 0; k < 6; ++k) v[i][j][k] = ts(i, j, k);
+    }
+};
+constexpr Table kTS{};
+
+// the Gram matrix is symmetric term by term
+__host__ __device__ constexpr bool symmetric() {
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j)
+            for (int k = 0; k < 6; ++k)
+                if (kTS.v[i][j][k] != kTS.v[j][i][k]) return false;
+    return true;
+}
+static_assert(symmetric(), "trace Gram table must be symmetric");
+// edge (0,1), first kind: the |grad lam_1|^2 coefficient of
+// int |-lam_0^2 grad lam_1 + lam_0 lam_1 grad lam_0|^2 is 4! (lam_0^4)
+static_assert(ts(0, 0, dot_index(1, 1)) == mono(0, 0, 0, 0), "edge self term");
+
+}  // namespace trace
+
 // Dunavant order-4 rule as generated by gaus_quad_tri(4): two 3-orbits
+// (the forcing and the port overlaps are defined on these points)
 __device__ __forceinline__ void dunavant4(int k, double& w, double& l1, double& l2, double& l3) {
     const double W[2] = {0.223381589678011, 0.109951743655322};
     const double A[2] = {0.108103018168070, 0.816847572980459};
@@ -43,140 +129,83 @@ __global__ void k_tri_terms(int64_t ntri, const int32_t* __restrict__ tv, const 
                             double2* __restrict__ force) {
     const int64_t it = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (it >= ntri) return;
-    double G[3][3];
+    double P[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int v = tv[k * ntri + it];
-        G[k][0] = xyz[v];
-        G[k][1] = xyz[V + v];
-        G[k][2] = xyz[2 * V + v];
+        P[k][0] = xyz[v];
+        P[k][1] = xyz[V + v];
+        P[k][2] = xyz[2 * V + v];
     }
-    // local frame (ASM/robinbc.py:135-148)
-    double e1[3], e2[3], zh[3], xh[3], yh[3];
+    double E1[3], E2[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        e1[d] = G[1][d] - G[0][d];
-        e2[d] = G[2][d] - G[0][d];
+        E1[d] = P[1][d] - P[0][d];
+        E2[d] = P[2][d] - P[0][d];
     }
-    zh[0] = e1[1] * e2[2] - e1[2] * e2[1];
-    zh[1] = e1[2] * e2[0] - e1[0] * e2[2];
-    zh[2] = e1[0] * e2[1] - e1[1] * e2[0];
-    double nz = sqrt(zh[0] * zh[0] + zh[1] * zh[1] + zh[2] * zh[2]);
-    double nx = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-    for (int d = 0; d < 3; ++d) {
-        zh[d] /= nz;
-        xh[d] = e1[d] / nx;
+    const double g11 = E1[0] * E1[0] + E1[1] * E1[1] + E1[2] * E1[2];
+    const double g22 = E2[0] * E2[0] + E2[1] * E2[1] + E2[2] * E2[2];
+    const double g12 = E1[0] * E2[0] + E1[1] * E2[1] + E1[2] * E2[2];
+    const double det = g11 * g22 - g12 * g12;
+    const double inv = 1.0 / det;
+    const double area = 0.5 * sqrt(det);
+    // grad lam_a . grad lam_b in dot_index order (00, 01, 02, 11, 12, 22)
+    const double dd[6] = {(g11 + g22 - 2.0 * g12) * inv, (g12 - g22) * inv, (g12 - g11) * inv,
+                          g22 * inv, -g12 * inv, g11 * inv};
+    // DOF length factors: |P1-P0|, |P2-P1|, |P2-P0|
+    const double e01 = sqrt(g11), e02 = sqrt(g22);
+    const double e12 = sqrt((P[2][0] - P[1][0]) * (P[2][0] - P[1][0]) + (P[2][1] - P[1][1]) * (P[2][1] - P[1][1]) +
+                            (P[2][2] - P[1][2]) * (P[2][2] - P[1][2]));
+    auto len = [&](int a, int b) { return (a + b == 1) ? e01 : ((a + b == 3) ? e12 : e02); };
+    double Ls[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const trace::Fn f = trace::fn(i);
+        Ls[i] = len(f.la, f.lb);
     }
-    yh[0] = zh[1] * xh[2] - zh[2] * xh[1];
-    yh[1] = zh[2] * xh[0] - zh[0] * xh[2];
-    yh[2] = zh[0] * xh[1] - zh[1] * xh[0];
-    double ny = sqrt(yh[0] * yh[0] + yh[1] * yh[1] + yh[2] * yh[2]);
-    for (int d = 0; d < 3; ++d) yh[d] /= ny;
-    double xs[3], ys[3];
-    for (int k = 0; k < 3; ++k) {
-        const double dx = G[k][0] - G[0][0], dy = G[k][1] - G[0][1], dz = G[k][2] - G[0][2];
-        xs[k] = xh[0] * dx + xh[1] * dy + xh[2] * dz;
-        ys[k] = yh[0] * dx + yh[1] * dy + yh[2] * dz;
-    }
-    const double x1 = xs[0], x2 = xs[1], x3 = xs[2], y1 = ys[0], y2 = ys[1], y3 = ys[2];
-    const double bb[3] = {y2 - y3, y3 - y1, y1 - y2};
-    const double cc[3] = {x3 - x2, x1 - x3, x2 - x1};
-    const double aa[3] = {x2 * y3 - y2 * x3, x3 * y1 - y3 * x1, x1 * y2 - y1 * x2};
-    double Ds[3][3];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Ds[i][j] = sqrt((xs[i] - xs[j]) * (xs[i] - xs[j]) + (ys[i] - ys[j]) * (ys[i] - ys[j]));
-    const double sgn = (x1 - x3) * (y2 - y1) - (x1 - x2) * (y3 - y1);
-    const double area = 0.5 * fabs(sgn);
-    auto dot2 = [&](int p, int q) { return bb[p] * bb[q] + cc[p] * cc[q]; };
-    auto AC = [&](int a, int b, int c, int d) { return area_coeff4(a, b, c, d) * area; };
-    const int lem[2][3] = {{0, 1, 0}, {1, 2, 2}};
-    double M[8][8];
-    for (int i = 0; i < 8; ++i)
-        for (int j = 0; j < 8; ++j) M[i][j] = 0.0;
-    const int tA = 1, tB = 2, tC = 3;
-    const double Lt1 = Ds[2][0], Lt2 = Ds[1][0];
-    for (int ei = 0; ei < 3; ++ei) {
-        const int ei1 = lem[0][ei], ei2 = lem[1][ei];
-        const double Li = Ds[ei1][ei2];
-        const int A = ei1 + 1, B = ei2 + 1;
-        for (int ej = 0; ej < 3; ++ej) {
-            const int ej1 = lem[0][ej], ej2 = lem[1][ej];
-            const double Lj = Ds[ej1][ej2];
-            const int C = ej1 + 1, D = ej2 + 1;
-            const double DAC = dot2(ei1, ej1), DAD = dot2(ei1, ej2), DBC = dot2(ei2, ej1), DBD = dot2(ei2, ej2);
-            const double LL = Li * Lj;
-            M[ei][ej] += LL * (AC(A, B, C, D) * DAC - AC(A, B, C, C) * DAD - AC(A, A, C, D) * DBC + AC(A, A, C, C) * DBD);
-            M[ei][ej + 4] += LL * (AC(A, B, D, D) * DAC - AC(A, B, C, D) * DAD - AC(A, A, D, D) * DBC + AC(A, A, C, D) * DBD);
-            M[ei + 4][ej] += LL * (AC(B, B, C, D) * DAC - AC(B, B, C, C) * DAD - AC(A, B, C, D) * DBC + AC(A, B, C, C) * DBD);
-            M[ei + 4][ej + 4] += LL * (AC(B, B, D, D) * DAC - AC(B, B, C, D) * DAD - AC(A, B, D, D) * DBC + AC(A, B, C, D) * DBD);
+    const double scale = area / 360.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (trace::kTS.v[i][j][k] != 0) acc += double(trace::kTS.v[i][j][k]) * dd[k];
+            mat[it * 64 + i * 8 + j] = Ls[i] * Ls[j] * scale * acc;
         }
-        const double FA = dot2(ei1, 2), FB = dot2(ei1, 0), FC = dot2(ei2, 2);
-        const double FD = dot2(ei2, 0), FE = dot2(ei1, 1), FF = dot2(ei2, 1);
-        M[ei][3] += Li * Lt1 * (AC(A, B, tA, tB) * FA - AC(A, B, tB, tC) * FB - AC(A, A, tA, tB) * FC + AC(A, A, tB, tC) * FD);
-        M[ei][7] += Li * Lt2 * (AC(A, B, tB, tC) * FB - AC(A, B, tC, tA) * FE - AC(A, A, tB, tC) * FD + AC(A, A, tC, tA) * FF);
-        M[3][ei] += Lt1 * Li * (AC(tA, tB, A, B) * FA - AC(tA, tB, A, A) * FC - AC(tB, tC, A, B) * FB + AC(tB, tC, A, A) * FD);
-        M[7][ei] += Lt2 * Li * (AC(tB, tC, A, B) * FB - AC(tB, tC, A, A) * FD - AC(tC, tA, A, B) * FE + AC(tC, tA, A, A) * FF);
-        M[ei + 4][3] += Li * Lt1 * (AC(B, B, tA, tB) * FA - AC(B, B, tB, tC) * FB - AC(A, B, tA, tB) * FC + AC(A, B, tB, tC) * FD);
-        M[ei + 4][7] += Li * Lt2 * (AC(B, B, tB, tC) * FB - AC(B, B, tC, tA) * FE - AC(A, B, tB, tC) * FD + AC(A, B, tC, tA) * FF);
-        M[3][ei + 4] += Lt1 * Li * (AC(tA, tB, B, B) * FA - AC(tA, tB, A, B) * FC - AC(tB, tC, B, B) * FB + AC(tB, tC, A, B) * FD);
-        M[7][ei + 4] += Lt2 * Li * (AC(tB, tC, B, B) * FB - AC(tB, tC, A, B) * FD - AC(tC, tA, B, B) * FE + AC(tC, tA, A, B) * FF);
     }
-    const double H1 = dot2(0, 2), H2 = dot2(0, 0), H3 = dot2(0, 1);
-    M[3][3] += Lt1 * Lt1 * (AC(tA, tB, tA, tB) * dot2(2, 2) - AC(tA, tB, tB, tC) * H1 - AC(tB, tC, tA, tB) * H1 + AC(tB, tC, tB, tC) * H2);
-    M[3][7] += Lt1 * Lt2 * (AC(tA, tB, tB, tC) * H1 - AC(tA, tB, tC, tA) * dot2(1, 2) - AC(tB, tC, tB, tC) * H2 + AC(tB, tC, tC, tA) * H3);
-    M[7][3] += Lt2 * Lt1 * (AC(tB, tC, tA, tB) * H1 - AC(tB, tC, tB, tC) * H2 - AC(tC, tA, tA, tB) * dot2(1, 2) + AC(tC, tA, tB, tC) * H3);
-    M[7][7] += Lt2 * Lt2 * (AC(tB, tC, tB, tC) * H2 - AC(tB, tC, tC, tA) * H3 - AC(tC, tA, tB, tC) * H3 + AC(tC, tA, tC, tA) * dot2(1, 1));
-    const double coef = 1.0 / ((2.0 * area) * (2.0 * area));
-    for (int i = 0; i < 8; ++i)
-        for (int j = 0; j < 8; ++j) mat[it * 64 + i * 8 + j] = M[i][j] * coef;
 
     if (uinc != nullptr && force != nullptr) {
         double2 fv[8];
+#pragma unroll
         for (int k = 0; k < 8; ++k) fv[k] = make_double2(0.0, 0.0);
-        const double signA = sgn > 0 ? -1.0 : (sgn < 0 ? 1.0 : 0.0);
         for (int q = 0; q < 6; ++q) {
-            double w, l1, l2, l3;
-            dunavant4(q, w, l1, l2, l3);
-            const double x = x1 * l1 + x2 * l2 + x3 * l3;
-            const double y = y1 * l1 + y2 * l2 + y3 * l3;
-            // U_inc in the triangle frame: (xhat . U, yhat . U)
+            double w, l[3];
+            dunavant4(q, w, l[0], l[1], l[2]);
             const double2 ux = uinc[(it * 6 + q) * 3 + 0], uy = uinc[(it * 6 + q) * 3 + 1],
                           uz = uinc[(it * 6 + q) * 3 + 2];
-            const double2 Ux = make_double2(xh[0] * ux.x + xh[1] * uy.x + xh[2] * uz.x,
-                                            xh[0] * ux.y + xh[1] * uy.y + xh[2] * uz.y);
-            const double2 Uy = make_double2(yh[0] * ux.x + yh[1] * uy.x + yh[2] * uz.x,
-                                            yh[0] * ux.y + yh[1] * uy.y + yh[2] * uz.y);
-            for (int ei = 0; ei < 3; ++ei) {
-                const int ei1 = lem[0][ei], ei2 = lem[1][ei];
-                const double Li = Ds[ei1][ei2];
-                const double Q = aa[ei2] + bb[ei2] * x + cc[ei2] * y;
-                const double Z = aa[ei1] + bb[ei1] * x + cc[ei1] * y;
-                const double A4 = 4 * area * area;
-                const double Q2 = Q / A4, Z2 = Z / A4, Ar2 = 1 / (2 * area);
-                const double e1x = (bb[ei1] * Q2 - bb[ei2] * Z2) * Z * Ar2;
-                const double e1y = (cc[ei1] * Q2 - cc[ei2] * Z2) * Z * Ar2;
-                const double e2x = (bb[ei1] * Q2 - bb[ei2] * Z2) * Q * Ar2;
-                const double e2y = (cc[ei1] * Q2 - cc[ei2] * Z2) * Q * Ar2;
-                const double s = signA * area * Li * w;
-                fv[ei].x += s * (e1x * Ux.x + e1y * Uy.x);
-                fv[ei].y += s * (e1x * Ux.y + e1y * Uy.y);
-                fv[ei + 4].x += s * (e2x * Ux.x + e2y * Uy.x);
-                fv[ei + 4].y += s * (e2x * Ux.y + e2y * Uy.y);
+            // E1 . U and E2 . U, then grad lam_g . U
+            const double2 a1 = make_double2(E1[0] * ux.x + E1[1] * uy.x + E1[2] * uz.x,
+                                            E1[0] * ux.y + E1[1] * uy.y + E1[2] * uz.y);
+            const double2 a2 = make_double2(E2[0] * ux.x + E2[1] * uy.x + E2[2] * uz.x,
+                                            E2[0] * ux.y + E2[1] * uy.y + E2[2] * uz.y);
+            double2 gu[3];
+            gu[1] = make_double2((g22 * a1.x - g12 * a2.x) * inv, (g22 * a1.y - g12 * a2.y) * inv);
+            gu[2] = make_double2((g11 * a2.x - g12 * a1.x) * inv, (g11 * a2.y - g12 * a1.y) * inv);
+            gu[0] = make_double2(-gu[1].x - gu[2].x, -gu[1].y - gu[2].y);
+            const double s = -area * w;   // signA = -1
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const trace::Fn f = trace::fn(i);
+                const double c0 = f.t0.c * l[f.t0.a] * l[f.t0.b];
+                const double c1 = f.t1.c * l[f.t1.a] * l[f.t1.b];
+                const double sl = s * Ls[i];
+                fv[i].x += sl * (c0 * gu[f.t0.g].x + c1 * gu[f.t1.g].x);
+                fv[i].y += sl * (c0 * gu[f.t0.g].y + c1 * gu[f.t1.g].y);
             }
-            const double Q = aa[1] + bb[1] * x + cc[1] * y;
-            const double Z = aa[0] + bb[0] * x + cc[0] * y;
-            const double FAr = 8 * area * area * area;
-            const double W = (aa[2] + bb[2] * x + cc[2] * y) / FAr;
-            const double W2 = Q * W;
-            const double f1x = Lt1 * (-bb[0] * W2 + bb[2] * Z * Q / FAr);
-            const double f1y = Lt1 * (-cc[0] * W2 + cc[2] * Z * Q / FAr);
-            const double f2x = Lt2 * (bb[0] * W2 - bb[1] * Z * W);
-            const double f2y = Lt2 * (cc[0] * W2 - cc[1] * Z * W);
-            const double s = signA * area * w;
-            fv[3].x += s * (f1x * Ux.x + f1y * Uy.x);
-            fv[3].y += s * (f1x * Ux.y + f1y * Uy.y);
-            fv[7].x += s * (f2x * Ux.x + f2y * Uy.x);
-            fv[7].y += s * (f2x * Ux.y + f2y * Uy.y);
         }
+#pragma unroll
         for (int k = 0; k < 8; ++k) force[it * 8 + k] = fv[k];
     }
 }
