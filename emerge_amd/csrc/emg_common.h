@@ -184,6 +184,10 @@ struct emg_ctx {
 
 namespace emg {
 
+// Takes the device address of the no-allocation far field's scratch array
+// for the current device, once per device (emg_farfield.hip).
+void ff_scratch_setup(int device);
+
 inline int fail(emg_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;

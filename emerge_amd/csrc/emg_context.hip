@@ -22,6 +22,7 @@ int emg_create(int device, emg_ctx** out) {
         delete c;
         return EMG_ERR_HIP;
     }
+    emg::ff_scratch_setup(device);
     *out = c;
     return EMG_OK;
 }

@@ -19,6 +19,7 @@
 // Cost: one FP64 sincos and 24 FMAs per (direction, sample): bound by the
 // FP64 vector pipe, not by memory.
 #include <algorithm>
+#include <mutex>
 
 #include "emg_common.h"
 
@@ -176,7 +177,69 @@ __global__ void k_ff_finish(int64_t N, int chunks, const double* __restrict__ pa
 
 inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
+// Launch plan of k_ff_sum: about 2048 blocks over (direction tiles x sample
+// chunks), whole LDS tiles per chunk, partial sums capped at 256 MB.
+struct FFPlan {
+    int64_t dtiles, chunks, per;
+};
+FFPlan ff_plan(int64_t M, int64_t N) {
+    FFPlan p{};
+    p.dtiles = (N + kFFThreads - 1) / kFFThreads;
+    int64_t chunks = std::max<int64_t>(1, (2048 + p.dtiles - 1) / p.dtiles);
+    chunks = std::min<int64_t>(chunks, (M + kFFTile - 1) / kFFTile);
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (int64_t(256) << 20) / (96 * N)));
+    chunks = std::min<int64_t>(chunks, 65535);
+    int64_t per = (M + chunks - 1) / chunks;
+    per = (per + kFFTile - 1) / kFFTile * kFFTile;
+    p.per = per;
+    p.chunks = (M + per - 1) / per;
+    return p;
+}
+
+// The kernels of one far-field evaluation on device buffers: emag[M],
+// emax[1], rec[kRec * M] and part[chunks * 12 * N] are scratch.
+void ff_launch(hipStream_t st, int64_t M, const double* E, const double* H, const double* pos, const double* wn,
+               int64_t N, const double* th, const double* ph, double k0, const FFPlan& p, double* emag, double* emax,
+               double* rec, double* part, double* Eout, double* Hout) {
+    const double2* E2 = reinterpret_cast<const double2*>(E);
+    const double2* H2 = reinterpret_cast<const double2*>(H);
+    k_ff_emag<<<blocks_for(M, 256), 256, 0, st>>>(M, E2, emag);
+    k_ff_max<<<1, 1024, 0, st>>>(M, emag, emax);
+    k_ff_prep<<<blocks_for(M, 256), 256, 0, st>>>(M, E2, H2, pos, wn, emag, emax, rec);
+    const dim3 grid{unsigned(p.dtiles), unsigned(p.chunks), 1u};
+    k_ff_sum<<<grid, kFFThreads, 0, st>>>(M, N, p.per, rec, th, ph, k0, part);
+    k_ff_finish<<<blocks_for(N, 256), 256, 0, st>>>(N, int(p.chunks), part, th, ph, k0,
+                                                    reinterpret_cast<double2*>(Eout),
+                                                    reinterpret_cast<double2*>(Hout));
+}
+
+// Scratch of emg_farfield_noalloc: a file-scope device array that every GPU
+// the library runs on holds until the process ends.  In host code its name
+// is only a stand-in; the device address comes from hipGetSymbolAddress,
+// taken once per device by emg::ff_scratch_setup (called by emg_create with
+// the device current).
+constexpr int64_t kScratchDoubles = int64_t(8) << 20;   // 64 MiB
+constexpr int64_t kScratchAlign = 32;                    // doubles (256 bytes)
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+inline int64_t round_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
+
 }  // namespace
+
+namespace emg {
+void ff_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch[device] = static_cast<double*>(p);
+    else (void)hipGetLastError();   // emg_farfield_noalloc reports the missing scratch
+}
+}  // namespace emg
 
 extern "C" int emg_farfield(emg_ctx* c, int64_t M, const double* E, const double* H, const double* pos,
                             const double* wn, int64_t N, const double* theta, const double* phi, double k0,
@@ -215,34 +278,44 @@ extern "C" int emg_farfield(emg_ctx* c, int64_t M, const double* E, const double
     EMG_HIP(c, hipMemcpyAsync(dwn.p, wn, sizeof(double) * 3 * M, hipMemcpyHostToDevice, st));
     EMG_HIP(c, hipMemcpyAsync(dth.p, theta, sizeof(double) * N, hipMemcpyHostToDevice, st));
     EMG_HIP(c, hipMemcpyAsync(dph.p, phi, sizeof(double) * N, hipMemcpyHostToDevice, st));
-    const double2* E2 = reinterpret_cast<const double2*>(dE.p);
-    const double2* H2 = reinterpret_cast<const double2*>(dH.p);
-    k_ff_emag<<<blocks_for(M, 256), 256, 0, st>>>(M, E2, emag.p);
-    k_ff_max<<<1, 1024, 0, st>>>(M, emag.p, emax.p);
-    k_ff_prep<<<blocks_for(M, 256), 256, 0, st>>>(M, E2, H2, dpos.p, dwn.p, emag.p, emax.p, rec.p);
-    EMG_HIP(c, hipGetLastError());
-
-    // launch shape: about 2048 blocks over (direction tiles x sample chunks),
-    // whole LDS tiles per chunk, partial sums capped at 256 MB
-    const int64_t dtiles = (N + kFFThreads - 1) / kFFThreads;
-    int64_t chunks = std::max<int64_t>(1, (2048 + dtiles - 1) / dtiles);
-    chunks = std::min<int64_t>(chunks, (M + kFFTile - 1) / kFFTile);
-    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (int64_t(256) << 20) / (96 * N)));
-    chunks = std::min<int64_t>(chunks, 65535);
-    int64_t per = (M + chunks - 1) / chunks;
-    per = (per + kFFTile - 1) / kFFTile * kFFTile;
-    chunks = (M + per - 1) / per;
-    EMG_HIP(c, part.alloc(size_t(chunks) * 12 * N));
+    const FFPlan plan = ff_plan(M, N);
+    EMG_HIP(c, part.alloc(size_t(plan.chunks) * 12 * N));
     EMG_HIP(c, dEo.alloc(6 * N));
     EMG_HIP(c, dHo.alloc(6 * N));
-    const dim3 grid{unsigned(dtiles), unsigned(chunks), 1u};
-    k_ff_sum<<<grid, kFFThreads, 0, st>>>(M, N, per, rec.p, dth.p, dph.p, k0, part.p);
-    k_ff_finish<<<blocks_for(N, 256), 256, 0, st>>>(N, int(chunks), part.p, dth.p, dph.p, k0,
-                                                    reinterpret_cast<double2*>(dEo.p),
-                                                    reinterpret_cast<double2*>(dHo.p));
+    ff_launch(st, M, dE.p, dH.p, dpos.p, dwn.p, N, dth.p, dph.p, k0, plan, emag.p, emax.p, rec.p, part.p, dEo.p,
+              dHo.p);
     EMG_HIP(c, hipGetLastError());
     EMG_HIP(c, hipMemcpyAsync(Eout, dEo.p, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, st));
     EMG_HIP(c, hipMemcpyAsync(Hout, dHo.p, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, st));
     EMG_HIP(c, hipStreamSynchronize(st));
+    return EMG_OK;
+}
+
+extern "C" int emg_farfield_noalloc(emg_ctx* c, int64_t M, const double* E, const double* H, const double* pos,
+                                    const double* wn, int64_t N, const double* theta, const double* phi, double k0,
+                                    double* Eout, double* Hout) {
+    EMG_CHECK_CTX(c);
+    if (M < 1 || N < 1)
+        return fail(c, EMG_ERR_ARG, "emg_farfield_noalloc: need at least one sample and one direction");
+    if (!E || !H || !pos || !wn || !theta || !phi || !Eout || !Hout)
+        return fail(c, EMG_ERR_ARG, "emg_farfield_noalloc: null array");
+    if (M > (int64_t(1) << 31) || N > (int64_t(1) << 31))
+        return fail(c, EMG_ERR_ARG, "emg_farfield_noalloc: more than 2^31 samples or directions");
+    double* base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch[c->device] : nullptr;
+    if (!base) return fail(c, EMG_ERR_HIP, "emg_farfield_noalloc: no scratch address for device %d", c->device);
+    const FFPlan plan = ff_plan(M, N);
+    // scratch layout: emax | emag[M] | rec[kRec M] | part[chunks 12 N], each 256-byte aligned
+    const int64_t o_emag = kScratchAlign;
+    const int64_t o_rec = o_emag + round_up(M, kScratchAlign);
+    const int64_t o_part = o_rec + round_up(kRec * M, kScratchAlign);
+    const int64_t need = o_part + plan.chunks * 12 * N;
+    if (need > kScratchDoubles)
+        return fail(c, EMG_ERR_ARG,
+                    "emg_farfield_noalloc: %lld samples x %lld directions need %.1f MiB of scratch, the limit is %lld MiB",
+                    (long long)M, (long long)N, double(need) * 8.0 / (1 << 20),
+                    (long long)(kScratchDoubles * 8 >> 20));
+    ff_launch(c->stream, M, E, H, pos, wn, N, theta, phi, k0, plan, base + o_emag, base, base + o_rec,
+              base + o_part, Eout, Hout);
+    EMG_HIP(c, hipGetLastError());
     return EMG_OK;
 }

@@ -318,6 +318,22 @@ int emg_field_points(emg_ctx* ctx, const double* field, int64_t P, const double*
 int emg_farfield(emg_ctx* ctx, int64_t M, const double* E, const double* H, const double* pos, const double* wn,
                  int64_t N, const double* theta, const double* phi, double k0, double* Eout, double* Hout);
 
+/* emg_farfield without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * pointer is a caller-owned DEVICE buffer with the layout of emg_farfield;
+ * the work is enqueued on the context's stream and the call returns without
+ * waiting (emg_synchronize waits).  Its scratch (per-sample magnitudes and
+ * records, per-chunk partial sums: 17 M + 12 N * chunks doubles, chunks as
+ * emg_farfield's launch plan, at most 8 per 256 directions) lives in one
+ * 64 MiB device array per GPU, held until the process ends; larger problems
+ * are rejected with EMG_ERR_ARG.  Calls that overlap on one GPU (any
+ * context) share that array and race.  Results equal emg_farfield's bit for
+ * bit (same kernels, same launch plan). */
+int emg_farfield_noalloc(emg_ctx* ctx, int64_t M, const double* E, const double* H, const double* pos,
+                         const double* wn, int64_t N, const double* theta, const double* phi, double k0,
+                         double* Eout, double* Hout);
+
 #ifdef __cplusplus
 }
 #endif
