@@ -38,7 +38,9 @@ def cpu_baseline(n_tets_sample: int = 100_000) -> dict:
     """The reference's CPU path on a bounded sample of the same workload,
     timed on the host: the oracle's C++/OpenMP restatement of its per-tet
     kernel (tet-parallel, as the reference's numba prange; all host threads)
-    plus scipy COO->CSR of E and B (single-threaded, as in the reference)."""
+    plus scipy COO->CSR of E and B (single-threaded, as in the reference).
+    Timed again with the element kernel on 4 threads, the reference's
+    default NUMBA_NUM_THREADS (src/emerge/__init__.py:42-48)."""
     from oracle import curlcurl as cc
     from oracle import port
     from scipy.sparse import coo_matrix
@@ -48,18 +50,28 @@ def cpu_baseline(n_tets_sample: int = 100_000) -> dict:
     m, f = prob.mesh, prob.basis
     args = (m.nodes, m.tets, m.tris, m.edges, m.edge_lengths, f.tet_to_field, m.tet_to_edge,
             cc.isotropic(1.0 / prob.inv_mu_r), cc.isotropic(prob.eps_r))
-    port.matrix_builder(*args)  # warm (thread pool, page faults)
-    t0 = time.perf_counter()
-    dE, dB, r, c, nth = port.matrix_builder(*args)
-    t1 = time.perf_counter()
     N = f.n_field
-    coo_matrix((dE, (r, c)), shape=(N, N)).tocsr()
-    coo_matrix((dB, (r, c)), shape=(N, N)).tocsr()
-    t2 = time.perf_counter()
-    dt = t2 - t0
-    return {"value": m.n_tets / dt / 1e6, "unit": "Mtets/s", "cores": int(nth), "kind": "port",
+
+    def run(threads):
+        port.set_threads(threads)
+        port.matrix_builder(*args)  # warm (thread pool, page faults)
+        t0 = time.perf_counter()
+        dE, dB, r, c, nth = port.matrix_builder(*args)
+        t1 = time.perf_counter()
+        coo_matrix((dE, (r, c)), shape=(N, N)).tocsr()
+        coo_matrix((dB, (r, c)), shape=(N, N)).tocsr()
+        t2 = time.perf_counter()
+        return int(nth), t1 - t0, t2 - t1
+
+    nth, tk, tc = run(0)
+    nth4, tk4, tc4 = run(4)
+    port.set_threads(0)
+    return {"value": m.n_tets / (tk + tc) / 1e6, "unit": "Mtets/s", "cores": nth, "kind": "port",
             "sample": f"{n}^3 jittered Kuhn cube ({m.n_tets} tets): C++/OpenMP port of the per-tet kernel "
-                      f"({nth} threads) {t1 - t0:.2f} s + scipy COO->CSR (1 thread) {t2 - t1:.2f} s"}
+                      f"({nth} threads) {tk:.2f} s + scipy COO->CSR (1 thread) {tc:.2f} s",
+            "at_reference_threads": {"value": m.n_tets / (tk4 + tc4) / 1e6, "cores": nth4,
+                                     "sample": f"same sample, per-tet kernel on {nth4} threads "
+                                               f"(NUMBA_NUM_THREADS default) {tk4:.2f} s + COO->CSR {tc4:.2f} s"}}
 
 
 def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method: str = "cocg",
@@ -100,9 +112,25 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     t0 = time.perf_counter()
     rm = sw.solve(mixed_f, precision="mixed", fblock=fblock, **kw_m)
     tm = time.perf_counter() - t0
-    print(f"[bench] sweep: mixed {tm:.2f} s", file=sys.stderr, flush=True)
+    print(f"[bench] sweep: mixed {tm:.2f} s; fp64 batched at the same block width", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    rb = sw.solve(mixed_f, precision="fp64", fblock=fblock, **kw)
+    tb = time.perf_counter() - t0
+    print(f"[bench] sweep: fp64 batched {tb:.2f} s", file=sys.stderr, flush=True)
     ok_p = int(np.sum(np.all(rp.relres <= 1e-8, axis=1)))
     ok_m = int(np.sum(np.all(rm.relres <= 1e-8, axis=1)))
+    ok_b = int(np.sum(np.all(rb.relres <= 1e-8, axis=1)))
+    # |dS| against the FP64 direct solve (oracle + SuperLU fixture of all 201
+    # points, tests/golden/sweeps.npz), where the points are on the grid
+    dS = {}
+    fx = os.path.join(ROOT, "tests", "golden", "sweeps.npz")
+    if os.path.exists(fx):
+        G = np.load(fx, allow_pickle=False)
+        gi = {float(f): i for i, f in enumerate(G["wr90_freqs"])}
+        for name, r, fr in (("mixed_batched", rm, mixed_f), ("fp64_batched", rb, mixed_f), ("fp64_plain", rp, plain_f)):
+            idx = [gi.get(float(f)) for f in fr]
+            if all(i is not None for i in idx):
+                dS[name] = float(np.abs(r.S - G["wr90_S"][idx]).max())
     out = {
         "workload": "WR-90 22.86x10.16x40 mm, Kuhn 11x5x18 (5,940 tets, 41,118 DOFs), TE10 ports, 8-12 GHz",
         "krylov": {"idr": f"IDR({idr_s})", "gmres": f"GMRES({restart})",
@@ -116,10 +144,19 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
                           "freq_points_per_s": round(ok_m / tm, 4), "fblock": fblock,
                           "fblock_tuning_us_per_system_iteration": tuning,
                           "fp64_escalated_systems": int(rm.fp64_escalated),
-                          "mean_iters_per_system": float(rm.iters.mean()), "max_relres": float(rm.relres.max())},
+                          "mean_iters_per_system": float(rm.iters.mean()), "max_relres": float(rm.relres.max()),
+                          "max_dS_vs_direct": dS.get("mixed_batched")},
+        "fp64_batched": {"points": len(mixed_f), "converged": ok_b, "seconds": round(tb, 3),
+                         "freq_points_per_s": round(ok_b / tb, 4), "fblock": fblock,
+                         "mean_iters_per_system": float(rb.iters.mean()), "max_relres": float(rb.relres.max()),
+                         "max_dS_vs_direct": dS.get("fp64_batched")},
     }
-    out["speedup_mixed_vs_fp64"] = round(out["mixed_batched"]["freq_points_per_s"]
-                                         / max(out["fp64_plain"]["freq_points_per_s"], 1e-30), 3)
+    out["fp64_plain"]["max_dS_vs_direct"] = dS.get("fp64_plain")
+    fp = max(out["fp64_plain"]["freq_points_per_s"], 1e-30)
+    fb = max(out["fp64_batched"]["freq_points_per_s"], 1e-30)
+    out["speedup_mixed_vs_fp64"] = round(out["mixed_batched"]["freq_points_per_s"] / fp, 3)
+    out["speedup_mixed_vs_fp64_batched"] = round(out["mixed_batched"]["freq_points_per_s"] / fb, 3)
+    out["speedup_fp64_batched_vs_plain"] = round(fb / fp, 3)
     if cpu_points > 0:
         from oracle import curlcurl as cc, fem
         from oracle import ports as oports

@@ -20,8 +20,12 @@ are absent), so the restatement is pinned independently:
 * COO -> CSR: scipy 1.15.3 itself (the version the reference's lock file pins);
 * Robin / port surface terms: 2-D quadrature of the tangential basis;
 * direct solve: scipy SuperLU with the reference's options;
-* S-parameters: closed forms of an empty rectangular waveguide
-  (|S21| -> 1, arg S21 = -beta L, S11 -> 0, reciprocity).
+* S-parameters and port excitation (tests/test_oracle_sparam.py):
+  arg S21 = -beta L of a TE10 section, the first-order ABC reflection
+  (beta - k0)/(beta + k0) exp(-2j beta L), the 1/sqrt(P_q) power law, the
+  transmission-line S-matrix of a parallel-plate TEM line between two
+  lumped ports (and a shorted one), and the discrete energy balance of an
+  internal lumped-port sheet.
 * far field (`oracle.farfield`): the closed-form far field of a Hertzian
   dipole from its exact near fields on a closed surface, the extinction
   theorem for a plane wave, and the divergence theorem for the weights.

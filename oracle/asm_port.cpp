@@ -294,6 +294,10 @@ void tet_matrices(const double xs[4], const double ys[4], const double zs[4], co
 
 extern "C" {
 
+// Threads of the following parallel regions (the reference's own default is
+// NUMBA_NUM_THREADS = 4, src/emerge/__init__.py:42-48); 0 = all host threads.
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+
 // COO data of E (curl-curl) and B (mass) for tets [0, T): entry (i, j) of
 // tet t at 400 t + 20 i + j, row tet_to_field[i, t], column tet_to_field[j, t]
 // (ASM/curlcurl.py:419-466).  Arrays are the oracle's: nodes (3, V) f64,

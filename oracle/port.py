@@ -42,8 +42,15 @@ def _load():
         i = ctypes.c_int64
         lib.oracle_matrix_builder.argtypes = [i, i, i, i] + [p] * 13
         lib.oracle_matrix_builder.restype = ctypes.c_int
+        lib.oracle_set_threads.argtypes = [ctypes.c_int]
+        lib.oracle_set_threads.restype = None
         _lib = lib
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the port (0 = all host threads)."""
+    _load().oracle_set_threads(int(n))
 
 
 def matrix_builder(nodes, tets, tris, edges, all_edge_lengths, tet_to_field, tet_to_edge, ur, er):

@@ -166,16 +166,26 @@ def sampled_rows_oracle(m, f, eps, mu, rows):
     return out
 
 
-def test_full_size_cube_sampled_rows(hip_ctx):
-    """60^3 jittered cube (1.296M tets, 8.3M DOFs): pattern and values of 300
-    random rows (all four DOF blocks) against the oracle, and global pattern
-    invariants (symmetric pattern, diagonal present, nnz per tet)."""
+@pytest.fixture(scope="module")
+def cube60(hip_ctx):
+    """The 60^3 jittered cube (1.296M tets, 8.3M DOFs, seed 0) assembled once
+    on the device for the full-size tests."""
     prob = cube_assembly(60, seed=0)
     m, f = prob.mesh, prob.basis
     print(f"full-size cube T={m.n_tets} N={f.n_field}", flush=True)
     dm = MeshOnDevice(hip_ctx, m.nodes, m.tets, f.tet_to_field, f.n_field)
     dm.assemble(prob.eps_r, prob.inv_mu_r)
     E, B = dm.csr()
+    yield prob, dm, E, B
+    del E, B
+
+
+def test_full_size_cube_sampled_rows(cube60):
+    """60^3 jittered cube: pattern and values of 300 random rows (all four
+    DOF blocks) against the oracle, and global pattern invariants (symmetric
+    pattern, diagonal present, nnz per tet)."""
+    prob, dm, E, B = cube60
+    m, f = prob.mesh, prob.basis
     assert E.nnz == dm.nnz and 270 < E.nnz / m.n_tets < 285
     rng = np.random.default_rng(3)
     rows = rng.choice(f.n_field, size=300, replace=False)
@@ -195,3 +205,31 @@ def test_full_size_cube_sampled_rows(hip_ctx):
     P.data = np.ones_like(P.data, dtype=np.float64)
     assert (abs(P - P.T)).nnz == 0
     assert np.all(E.diagonal() != 0)
+
+
+def test_full_size_cube_every_row_against_fixtures(cube60):
+    """All 8.3M rows of the 60^3 cube against committed oracle fixtures
+    (tests/golden/make_cube60.py): SHA-256 of scipy's indptr / indices
+    (bit-exact pattern), and block checksums sum_i u_i (A v)_i over blocks of
+    4096 rows for seeded v, u, each within 1e-12 of the block's absolute sum
+    sum_i sum_j |A_ij| |v_j| (E and B)."""
+    import hashlib
+    import os
+    prob, dm, E, B = cube60
+    G = np.load(os.path.join(os.path.dirname(__file__), "golden", "cube60.npz"), allow_pickle=False)
+    assert int(G["n_field"]) == prob.basis.n_field and int(G["nnz"]) == E.nnz
+    assert hashlib.sha256(E.indptr.astype(np.int32).tobytes()).hexdigest() == str(G["sha_indptr"])
+    assert hashlib.sha256(E.indices.astype(np.int32).tobytes()).hexdigest() == str(G["sha_indices"])
+    N = prob.basis.n_field
+    rng = np.random.default_rng(int(G["seed_v"]))
+    v = rng.normal(size=N) + 1j * rng.normal(size=N)
+    u = np.where(np.random.default_rng(int(G["seed_u"])).random(N) < 0.5, -1.0, 1.0)
+    blk = int(G["block"])
+    nb = (N + blk - 1) // blk
+    pad = nb * blk - N
+    for name, A in (("E", E), ("B", B)):
+        y = A @ v
+        c = np.concatenate([y * u, np.zeros(pad, dtype=y.dtype)]).reshape(nb, blk).sum(axis=1)
+        err = np.abs(c - G["c" + name]) / G["a" + name]
+        print(f"{name}: max block error {err.max():.2e}", flush=True)
+        assert err.max() <= 1e-12, (name, int(err.argmax()), err.max())
