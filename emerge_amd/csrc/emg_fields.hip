@@ -23,7 +23,8 @@
 //   curl E += [-m1 L1 (g2 x (F3 g1 - F1 g3) + 2 F2 g3 x g1)
 //              + m2 L2 (g3 x (F2 g1 - F1 g2) + 2 F3 g2 x g1)] / (216 V^3)
 // (the reference writes the curl expanded; this is the same polynomial).
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -302,10 +303,10 @@ static int build_grid(emg_ctx* c) {
     k_grid_count<<<bt, 256, 0, st>>>(T, g, c->xyz.p, V, c->tets.p, cnt.p);
     EMG_HIP(c, hipGetLastError());
     size_t sb = 0;
-    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cnt.p, off.p, int(T), st));
+    EMG_HIP(c, rocprim::exclusive_scan(nullptr, sb, cnt.p, off.p, int64_t(0), size_t(T), rocprim::plus<int64_t>(), st));
     DBuf<unsigned char> tmp;
     EMG_HIP(c, tmp.alloc(std::max<size_t>(sb, 1)));
-    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp.p, sb, cnt.p, off.p, int(T), st));
+    EMG_HIP(c, rocprim::exclusive_scan(tmp.p, sb, cnt.p, off.p, int64_t(0), size_t(T), rocprim::plus<int64_t>(), st));
     int64_t last_off = 0, last_cnt = 0;
     EMG_HIP(c, hipMemcpyAsync(&last_off, off.p + (T - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
     EMG_HIP(c, hipMemcpyAsync(&last_cnt, cnt.p + (T - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -324,11 +325,11 @@ static int build_grid(emg_ctx* c) {
     int bits = 1;
     while ((int64_t(1) << bits) < ncells) ++bits;
     size_t rb = 0;
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, rb, k_in.p, k_out.p, v_in.p, c->grid_tets.p, int(total), 0,
-                                                  bits, st));
+    EMG_HIP(c, rocprim::radix_sort_pairs(nullptr, rb, k_in.p, k_out.p, v_in.p, c->grid_tets.p, size_t(total), 0u,
+                                              unsigned(bits), st));
     EMG_HIP(c, tmp.alloc(std::max<size_t>(rb, 1)));
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp.p, rb, k_in.p, k_out.p, v_in.p, c->grid_tets.p, int(total), 0,
-                                                  bits, st));
+    EMG_HIP(c, rocprim::radix_sort_pairs(tmp.p, rb, k_in.p, k_out.p, v_in.p, c->grid_tets.p, size_t(total), 0u,
+                                              unsigned(bits), st));
     EMG_HIP(c, c->grid_ptr.alloc(ncells + 1));
     k_grid_ptr<<<unsigned((ncells + 1 + 255) / 256), 256, 0, st>>>(ncells, k_out.p, total, c->grid_ptr.p);
     EMG_HIP(c, hipGetLastError());

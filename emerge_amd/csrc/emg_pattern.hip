@@ -7,7 +7,8 @@
 // (row, col) pairs in row-major, column-sorted order, explicit zeros kept.
 // That is exactly a stable radix sort of packed (row<<nb | col) keys followed
 // by run-length unique.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 
@@ -222,10 +223,10 @@ extern "C" int emg_pattern_build(emg_ctx* c, int64_t* nnz_out) {
     k_coo_keys<<<grid_for(MI, 256), 256, 0, st>>>(c->t2f.p, T, nb, kin.p, vin.p);
     EMG_HIP(c, hipGetLastError());
     size_t tmp_bytes = 0;
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin.p, kout.p, vin.p, vout.p, M, 0, 2 * nb, st));
+    EMG_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin.p, kout.p, vin.p, vout.p, size_t(M), 0u, unsigned(2 * nb), st));
     DBuf<unsigned char> tmp;
     EMG_HIP(c, tmp.alloc(tmp_bytes));
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, kin.p, kout.p, vin.p, vout.p, M, 0, 2 * nb, st));
+    EMG_HIP(c, rocprim::radix_sort_pairs(tmp.p, tmp_bytes, kin.p, kout.p, vin.p, vout.p, size_t(M), 0u, unsigned(2 * nb), st));
     kin.release();
     vin.release();
     DBuf<int32_t> head, slotp1;
@@ -234,12 +235,12 @@ extern "C" int emg_pattern_build(emg_ctx* c, int64_t* nnz_out) {
     k_heads<<<grid_for(M, 256), 256, 0, st>>>(kout.p, M, head.p);
     EMG_HIP(c, hipGetLastError());
     size_t scan_bytes = 0;
-    EMG_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, head.p, slotp1.p, M, st));
+    EMG_HIP(c, rocprim::inclusive_scan(nullptr, scan_bytes, head.p, slotp1.p, size_t(M), rocprim::plus<int32_t>(), st));
     if (scan_bytes > tmp.n) {
         tmp.release();
         EMG_HIP(c, tmp.alloc(scan_bytes));
     }
-    EMG_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp.p, scan_bytes, head.p, slotp1.p, M, st));
+    EMG_HIP(c, rocprim::inclusive_scan(tmp.p, scan_bytes, head.p, slotp1.p, size_t(M), rocprim::plus<int32_t>(), st));
     int32_t nnz32 = 0;
     EMG_HIP(c, hipMemcpyAsync(&nnz32, slotp1.p + (M - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
     EMG_HIP(c, hipStreamSynchronize(st));
@@ -270,12 +271,12 @@ extern "C" int emg_pattern_build(emg_ctx* c, int64_t* nnz_out) {
     k_inc_keys<<<grid_for(MI, 256), 256, 0, st>>>(c->t2f.p, T, ikin.p, ivin.p);
     EMG_HIP(c, hipGetLastError());
     size_t itmp = 0;
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, itmp, ikin.p, ikout.p, ivin.p, ivout.p, MI, 0, nb, st));
+    EMG_HIP(c, rocprim::radix_sort_pairs(nullptr, itmp, ikin.p, ikout.p, ivin.p, ivout.p, size_t(MI), 0u, unsigned(nb), st));
     if (itmp > tmp.n) {
         tmp.release();
         EMG_HIP(c, tmp.alloc(itmp));
     }
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp.p, itmp, ikin.p, ikout.p, ivin.p, ivout.p, MI, 0, nb, st));
+    EMG_HIP(c, rocprim::radix_sort_pairs(tmp.p, itmp, ikin.p, ikout.p, ivin.p, ivout.p, size_t(MI), 0u, unsigned(nb), st));
     DBuf<int32_t> incptr;
     EMG_HIP(c, incptr.alloc(N + 1));
     k_rowptr<uint32_t><<<grid_for(MI + 1, 256), 256, 0, st>>>(ikout.p, MI, N, incptr.p);

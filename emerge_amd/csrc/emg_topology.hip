@@ -11,11 +11,12 @@
 // at :26); edge lengths sqrt(sum((n[:,e0]-n[:,e1])**2)) (:497).
 //
 // Keys: an edge (a<b) is a*V + b, a triangle (a<b<c) is (a*V + b)*V + c
-// (64-bit, V < 2^21); hipcub radix sorts, a head-flag scan compacts the
+// (64-bit, V < 2^21); rocPRIM radix sorts, a head-flag scan compacts the
 // unique keys (so they come out in lexicographic order), and every map is a
 // binary search in the sorted keys.  tri_to_tet takes the min / max tet id
 // per triangle with atomics (the reference's first / second in tet order).
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <climits>
 #include <cstdint>
@@ -183,12 +184,12 @@ int unique_keys(emg_ctx* c, int64_t n, DBuf<uint64_t>& in, DBuf<uint64_t>& out, 
     EMG_HIP(c, head.alloc(n));
     EMG_HIP(c, pos.alloc(n));
     size_t sb = 0, cb = 0;
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sb, in.p, sorted.p, int(n), 0, 64, s));
-    EMG_HIP(c, hipcub::DeviceScan::InclusiveSum(nullptr, cb, head.p, pos.p, int(n), s));
+    EMG_HIP(c, rocprim::radix_sort_keys(nullptr, sb, in.p, sorted.p, size_t(n), 0u, 64u, s));
+    EMG_HIP(c, rocprim::inclusive_scan(nullptr, cb, head.p, pos.p, size_t(n), rocprim::plus<int32_t>(), s));
     EMG_HIP(c, tmp.alloc(std::max(sb, cb)));
-    EMG_HIP(c, hipcub::DeviceRadixSort::SortKeys(tmp.p, sb, in.p, sorted.p, int(n), 0, 64, s));
+    EMG_HIP(c, rocprim::radix_sort_keys(tmp.p, sb, in.p, sorted.p, size_t(n), 0u, 64u, s));
     k_heads<<<unsigned((n + 255) / 256), 256, 0, s>>>(n, sorted.p, head.p);
-    EMG_HIP(c, hipcub::DeviceScan::InclusiveSum(tmp.p, cb, head.p, pos.p, int(n), s));
+    EMG_HIP(c, rocprim::inclusive_scan(tmp.p, cb, head.p, pos.p, size_t(n), rocprim::plus<int32_t>(), s));
     int32_t hc = 0;
     EMG_HIP(c, hipMemcpyAsync(&hc, pos.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     EMG_HIP(c, hipStreamSynchronize(s));
@@ -388,10 +389,10 @@ extern "C" int emg_pec_keep(emg_ctx* c, int64_t n_pec, const int32_t* pec_tris, 
     }
     EMG_HIP(c, hipGetLastError());
     size_t sb = 0;
-    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, keep.p, pos.p, int(N), s));
+    EMG_HIP(c, rocprim::exclusive_scan(nullptr, sb, keep.p, pos.p, int32_t(0), size_t(N), rocprim::plus<int32_t>(), s));
     DBuf<unsigned char> tmp;
     EMG_HIP(c, tmp.alloc(std::max<size_t>(sb, 1)));
-    EMG_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp.p, sb, keep.p, pos.p, int(N), s));
+    EMG_HIP(c, rocprim::exclusive_scan(tmp.p, sb, keep.p, pos.p, int32_t(0), size_t(N), rocprim::plus<int32_t>(), s));
     int32_t last_pos = 0, last_keep = 0;
     EMG_HIP(c, hipMemcpyAsync(&last_pos, pos.p + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     EMG_HIP(c, hipMemcpyAsync(&last_keep, keep.p + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
