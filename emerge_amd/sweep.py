@@ -29,6 +29,26 @@ C0 = 299792458
 EPS0 = 8.854187818814e-12
 
 
+PECLIM = 1e7   # settings.mw_3d_peclim (E/settings.py:24): bulk conductors above it are PEC
+
+
+def conductor_split(cond, peclim: float = PECLIM):
+    """Per-tet classification of a conductivity (T,) or (3, 3, T) as the
+    reference makes it (ASM/assembler.py:240-288, on cond[0, 0]): tets above
+    `peclim` are perfect conductors (all their DOFs eliminated), the others
+    with cond > 0 make the permittivity frequency-dependent.  Returns
+    (pec_tets bool (T,), lossy conductivity with the PEC tets zeroed)."""
+    c = np.asarray(cond)
+    c00 = (c if c.ndim == 1 else c[0, 0]).real
+    pec = c00 > peclim
+    lossy = np.array(c, dtype=np.complex128, copy=True)
+    if lossy.ndim == 1:
+        lossy[pec] = 0
+    else:
+        lossy[:, :, pec] = 0
+    return pec, lossy
+
+
 def lossy_permittivity(eps_r, tand, cond, freq: float) -> np.ndarray:
     """Complex relative permittivity with loss tangent and conductivity (S/m)
     at `freq`: ``eps_r (1 - j tand) - j cond / (w eps0)``, w = 2 pi freq
@@ -103,7 +123,8 @@ class FrequencySweep:
     """Device-resident E, B, surface terms and forcing of one mesh."""
 
     def __init__(self, mesh, basis, eps_r, inv_mu_r, boundary_conditions: list,
-                 device: int = 0, ctx: Context | None = None, dof_order: str = "spatial", mu_r=None):
+                 device: int = 0, ctx: Context | None = None, dof_order: str = "spatial", mu_r=None,
+                 cond=None, peclim: float = PECLIM):
         """Materials per tet: eps_r a complex scalar (T,) or tensor (3, 3, T);
         inv_mu_r complex scalars 1/mu_r (T,), or mu_r a tensor (3, 3, T) (the
         reference's ``ur``, inverted on the device) in its place.  Any of them
@@ -112,7 +133,31 @@ class FrequencySweep:
         re-assembled before its systems are solved, as the reference does
         for frequency-dependent or conductive materials
         (ASM/assembler.py:222-252; conductivity as ``eps_r - 1j*cond/(w eps0)``,
-        ``lossy_permittivity``)."""
+        ``lossy_permittivity``).  cond: per-tet conductivity in S/m, (T,) or
+        (3, 3, T), as the reference's ``cond`` array: tets above `peclim` are
+        perfect conductors (their DOFs are eliminated with the PEC faces, and
+        their permittivity plays no part), the other conducting tets add
+        -j cond / (w eps0) to eps_r at every frequency (the sweep is then
+        frequency-dependent; ASM/assembler.py:240-288)."""
+        self.conductor_tets = None
+        if cond is not None:
+            pec_t, lossy = conductor_split(cond, peclim)
+            self.conductor_tets = np.nonzero(pec_t)[0]
+            if pec_t.any():
+                boundary_conditions = list(boundary_conditions) + [
+                    bcs.PEC(np.unique(mesh.tet_to_tri[:, pec_t].reshape(-1)))]
+            if np.any(lossy != 0):
+                base = eps_r
+
+                def eps_r(freq, base=base, sig=lossy):
+                    e = np.asarray(base(freq) if callable(base) else base, dtype=np.complex128)
+                    if e.ndim != sig.ndim:
+                        eye = np.eye(3)[:, :, None]
+                        if e.ndim == 1:
+                            e = e[None, None, :] * eye
+                        else:
+                            sig = sig[None, None, :] * eye
+                    return e - 1j * sig / (2 * np.pi * freq * EPS0)
         if dof_order not in ("spatial", "natural"):
             raise ValueError("dof_order must be 'spatial' or 'natural'")
         if (inv_mu_r is None) == (mu_r is None):
@@ -519,53 +564,75 @@ class FrequencySweep:
         return out
 
     # ------------------------------------------------------------------ solve
-    def batch_coefficients(self, freqs: np.ndarray):
+    def batch_coefficients(self, freqs: np.ndarray, groups=None):
         """k2, gamma [batch][bc] and rhs coefficients [batch][port] for the
-        (frequency, active port) pairs in frequency-major order."""
+        (frequency, excitation) pairs in frequency-major order; an excitation
+        is one port, or with `groups` (lists of port indices) every port of
+        a group at once."""
         freqs = np.asarray(freqs, dtype=np.float64)
         nb = len(self.surface)
         npt = len(self.ports)
+        groups = [[j] for j in range(npt)] if groups is None else groups
+        ng = len(groups)
         k0 = 2 * np.pi * freqs / C0
-        k2 = np.repeat(k0 ** 2, npt)
-        gam = np.zeros((freqs.shape[0] * npt, max(nb, 1)), dtype=np.complex128)
-        coef = np.zeros((freqs.shape[0] * npt, npt), dtype=np.complex128)
+        k2 = np.repeat(k0 ** 2, ng)
+        gam = np.zeros((freqs.shape[0] * ng, max(nb, 1)), dtype=np.complex128)
+        coef = np.zeros((freqs.shape[0] * ng, npt), dtype=np.complex128)
         for i, k in enumerate(k0):
             g = np.array([b.abc2_coefficient(k) if kind == "abc2" else b.get_gamma(k)
                           for kind, b in self.surface], dtype=np.complex128)
-            for j, p in enumerate(self.ports):
-                row = i * npt + j
+            for jg, grp in enumerate(groups):
+                row = i * ng + jg
                 gam[row, :nb] = g
-                coef[row, j] = p.uinc_factor(k)
+                for j in grp:
+                    coef[row, j] = self.ports[j].uinc_factor(k)
         return np.ascontiguousarray(k2), np.ascontiguousarray(gam[:, :nb] if nb else gam), np.ascontiguousarray(coef)
 
-    def s_matrix(self, overlaps: np.ndarray, freqs: np.ndarray, caps: bool = True) -> np.ndarray:
+    def s_matrix(self, overlaps: np.ndarray, freqs: np.ndarray, caps: bool = True, groups=None,
+                 blocks=None) -> np.ndarray:
         """S from the device overlaps, receiving port q, active port j
         (MW/microwave_3d.py:1193-1211, 1250-1303; MW/sparam.py:71-137).
         Mode-overlap port: field_p = (conj(q_m) O - [q==j] |q_m|^2 N_q) / 2Z,
         mode_p = |q_m|^2 N_q / 2Z, S_qj = field_p / mode_p / sqrt(P_j).
         Voltage-line port (O = mean line voltage V, V_q = sqrt(2 P_q Z0_q)):
-        S_qj = ((V - [q==j] V_q) / V_q) / sqrt(P_j)."""
+        S_qj = ((V - [q==j] V_q) / V_q) / sqrt(P_j).  With port `groups`
+        excited together and independent `blocks` of ports (one port of each
+        group per block), the active port seen by q is the one of q's block;
+        entries between blocks stay 0 and the caps act per block."""
         npt = len(self.ports)
+        groups = [[j] for j in range(npt)] if groups is None else groups
+        blocks = [list(range(npt))] if blocks is None else blocks
+        block_of = np.empty(npt, dtype=np.int64)
+        for ib, bl in enumerate(blocks):
+            block_of[bl] = ib
+        ng = len(groups)
         S = np.zeros((freqs.shape[0], npt, npt), dtype=np.complex128)
         for i, f in enumerate(freqs):
             k0 = 2 * np.pi * f / C0
-            for j, act in enumerate(self.ports):
+            for jg, grp in enumerate(groups):
+                row = i * ng + jg
                 for q, prb in enumerate(self.ports):
+                    act_in = [j for j in grp if block_of[j] == block_of[q]]
+                    if len(act_in) != 1:
+                        continue
+                    j = act_in[0]
+                    act = self.ports[j]
                     if getattr(prb, "v_integration", False):
-                        V = overlaps[i * npt + j, q]
+                        V = overlaps[row, q]
                         a = prb.voltage
                         b = V - a if q == j else V
                         S[i, q, j] = (b / a) / np.sqrt(act.power)
                         continue
                     qm = prb.mode_factor(k0)
                     Zm = prb.Zmode(k0)
-                    O = overlaps[i * npt + j, q]
+                    O = overlaps[row, q]
                     N = self.port_norms[q]
                     fp = (np.conj(qm) * O - (1.0 if q == j else 0.0) * abs(qm) ** 2 * N) / (2 * Zm)
                     mp = abs(qm) ** 2 * N / (2 * Zm)
                     S[i, q, j] = (fp / mp) / np.sqrt(act.power)
             if caps:
-                S[i] = apply_s_caps(S[i])
+                for bl in blocks:
+                    S[i][np.ix_(bl, bl)] = apply_s_caps(S[i][np.ix_(bl, bl)])
         return S
 
     def needs_safe32(self, freqs) -> bool:
@@ -632,7 +699,7 @@ class FrequencySweep:
     def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock=32, restart: int | None = None,
               max_cycles: int = 200, keep_fields: bool = True, method: str = "cocg", idr_s: int = 4,
               inner_rel32: float | None = None, precond: str = "aux", fallback: bool = True,
-              safe32="auto") -> SweepResult:
+              safe32="auto", groups=None, blocks=None) -> SweepResult:
         """Solve every (frequency, active port) system.  method 'gmres' =
         GMRES(restart); 'idr' = IDR(idr_s) with a budget of `restart`
         matvecs per FP64 refinement cycle; 'cocg' = COCG on the symmetric
@@ -649,7 +716,10 @@ class FrequencySweep:
         SpMV and the gradient-space correction in FP64; 'auto' turns it on
         when ``needs_safe32`` says the mesh is fine against the wavelength.
         precision: 'fp64', 'mixed' (complex64 inner solves) or 'auto'
-        (``choose_precision``)."""
+        (``choose_precision``).  groups / blocks: excite the ports of each
+        group together (one system per frequency and group) and extract S per
+        block of independent ports (``s_matrix``); fields are then keyed by
+        (frequency index, group index)."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
         if precision == "auto":
             precision = self.choose_precision(freqs)
@@ -666,7 +736,7 @@ class FrequencySweep:
         if safe32 == "auto":
             safe32 = self.needs_safe32(freqs)
         self.ctx.call("emg_solver_set_safe32", int(bool(safe32)))
-        npt = len(self.ports)
+        npt = len(self.ports) if groups is None else len(groups)
         if self.frequency_dependent:
             # one frequency per device batch: each has its own E and B
             fblock = max(1, npt)
@@ -675,7 +745,7 @@ class FrequencySweep:
                                           idr_s=idr_s, inner_rel32=inner_rel32)
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s),
                       float(inner_rel32))
-        k2, gam, coef = self.batch_coefficients(freqs)
+        k2, gam, coef = self.batch_coefficients(freqs, groups)
         self._escalated = 0
         run = dict(fblock=fblock, tol=tol, restart=restart, max_cycles=max_cycles, keep_fields=keep_fields)
         prec = {"fp64": 0, "mixed": 1}[precision]
@@ -695,12 +765,13 @@ class FrequencySweep:
                 None if parts[0][j] is None else np.concatenate([q[j] for q in parts]) for j in range(5))
         fields = {}
         if x is not None:
+            keys = [p.port_number for p in self.ports] if groups is None else list(range(npt))
             for i in range(freqs.shape[0]):
-                for j, p in enumerate(self.ports):
+                for j, key in enumerate(keys):
                     full = np.zeros(self.basis.n_field, dtype=np.complex128)
                     full[self.solve_ids] = x[i * npt + j]
-                    fields[(i, p.port_number)] = full
-        S = self.s_matrix(ov, freqs) if ov is not None else None
+                    fields[(i, key)] = full
+        S = self.s_matrix(ov, freqs, groups=groups, blocks=blocks) if ov is not None else None
         return SweepResult(freqs, self.ports, fields, relres.reshape(-1, npt), iters.reshape(-1, npt), precision,
                            S=S, fp64_fallback=fell_back.reshape(-1, npt), fp64_escalated=self._escalated)
 
