@@ -29,9 +29,12 @@ from .quadrature import tet_coefficients
 C0 = 299792458
 
 
-def assemble_freq_matrix(mesh, maps: dict, E, B, pec_tris, robin_bcs, frequency: float):
+def assemble_freq_matrix(mesh, maps: dict, E, B, pec_tris, robin_bcs, frequency: float, cond=None,
+                         peclim: float = 1e7, tet_to_field=None):
     """Returns (K, b, port_vectors, solve_ids).  ``robin_bcs`` are oracle.ports
-    objects; ports (with a ``port_number``) are excited."""
+    objects; ports (with a ``port_number``) are excited.  ``cond`` (3, 3, T):
+    every tet with cond[0, 0] > peclim is a conductor whose 20 DOFs are
+    eliminated (ASM/assembler.py:280-288; settings.mw_3d_peclim = 1e7)."""
     K0 = 2 * np.pi * frequency / C0
     K = (E - B * (K0**2)).tocsr()
     N = E.shape[0]
@@ -39,6 +42,10 @@ def assemble_freq_matrix(mesh, maps: dict, E, B, pec_tris, robin_bcs, frequency:
     port_vectors = {bc.port_number: np.zeros(N, dtype=np.complex128)
                     for bc in robin_bcs if hasattr(bc, "port_number")}
     pec_ids = []
+    if cond is not None:
+        for itet in range(mesh.n_tets):
+            if cond[0, 0, itet] > peclim:
+                pec_ids.extend(tet_to_field[:, itet])
     tri_ids = np.asarray(pec_tris, dtype=np.int64)
     for ii in list(mesh.tri_to_edge[:, tri_ids].flatten()):
         pec_ids.extend(list(maps["edge_to_field"][:, ii]))

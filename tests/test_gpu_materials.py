@@ -190,3 +190,51 @@ def test_refresh_needs_current_reduction():
     ctx.pattern_build()
     with pytest.raises(EmgError):
         ctx.call("emg_system_refresh")
+
+
+def _metal_case():
+    """Copper post (cond 5.8e7 S/m > 1e7: a perfect conductor) across the
+    guide's narrow side at x in (7.5, 10.4) mm, z in (10, 14.5) mm, and the
+    lossy block (cond 0.2 S/m) of the other tests behind it."""
+    c = wg_case.case()
+    m = c["mesh"]
+    T = m.n_tets
+    cen = tet_centers(m)
+    post = (cen[0] > 7.5e-3) & (cen[0] < 10.4e-3) & (cen[2] > 10e-3) & (cen[2] < 14.5e-3)
+    blk = _block(m)
+    cond = np.where(post, 5.8e7, np.where(blk, 0.2, 0.0))
+    eps = np.where(blk, 2.5 * (1 - 0.005j), 1.0 + 0j).astype(np.complex128)
+    return c, m, T, post, cond, eps
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_perfect_conductor_tets_and_lossy_block(precision):
+    """Tets above the conductivity limit lose all 20 DOFs, the lossy block
+    makes the sweep frequency-dependent (ASM/assembler.py:240-288), against
+    the oracle assembled per frequency with eps_r - j cond / (w eps0) on
+    every tet and the conductor elimination, + SuperLU."""
+    print(f"metal post precision={precision}", flush=True)
+    c, m, T, post, cond, eps = _metal_case()
+    assert post.sum() >= 6
+    sw = FrequencySweep(m, c["basis"], eps, np.ones(T, dtype=np.complex128), c["prod_bcs"], cond=cond)
+    assert sw.frequency_dependent and np.array_equal(sw.conductor_tets, np.nonzero(post)[0])
+    res = sw.solve(FREQS, precision=precision, tol=1e-8)
+    assert np.all(res.relres <= 1e-8), res.relres
+    cond3 = cc.isotropic(cond.astype(np.complex128))
+    for i, f in enumerate(FREQS):
+        er = lossy_permittivity(eps, 0.0, cond, float(f))
+        E, B = cc.tet_mass_stiffness_matrices(m, c["basis"].tet_to_field, cc.isotropic(er),
+                                              cc.isotropic(np.ones(T, dtype=np.complex128)))
+        K, b, pv, ids = fem.assemble_freq_matrix(m, c["maps"], E, B, c["pec"], c["o_ports"], float(f), cond=cond3,
+                                                 tet_to_field=c["basis"].tet_to_field)
+        assert np.array_equal(np.sort(sw.solve_ids), ids)
+        fields = fem.solve_all_ports(K, b, pv, ids)
+        S = fem.s_parameters(m, c["basis"].tet_to_field, fields, c["o_ports"], 2 * np.pi * f / fem.C0)
+        for p in sw.ports:
+            x = res.fields[(i, p.port_number)]
+            bb = b + pv[p.port_number]
+            rr = np.linalg.norm((bb - K.tocsr() @ x)[ids]) / np.linalg.norm(bb[ids])
+            assert rr <= 1e-8, (f, p.port_number, rr)
+        dS = np.abs(res.S[i] - S).max()
+        print(f"f={f:.3e} dS={dS:.2e} |S11|={abs(S[0, 0]):.3f} iters={res.iters[i].tolist()}", flush=True)
+        assert dS <= 1e-6, (f, dS)

@@ -82,3 +82,20 @@ def test_lossy_permittivity():
     e = lossy_permittivity(np.array([4.0, 4.0]), np.array([0.0, 0.01]), np.array([w * EPS0, 0.0]), f)
     assert abs(e[0] - (4 - 1j)) < 1e-14
     assert e[1] == 4 * (1 - 0.01j)
+
+
+def test_conductor_split_follows_the_reference_limit():
+    """cond[0, 0] > 1e7 S/m: perfect conductor; 0 < cond <= 1e7: lossy
+    (ASM/assembler.py:240-288, E/settings.py:24)."""
+    from emerge_amd.sweep import PECLIM, conductor_split
+    assert PECLIM == 1e7
+    c = np.array([0.0, 0.2, 1e7, 1.0000001e7, 5.8e7])
+    pec, lossy = conductor_split(c)
+    np.testing.assert_array_equal(pec, [False, False, False, True, True])
+    np.testing.assert_array_equal(lossy, [0.0, 0.2, 1e7, 0.0, 0.0])
+    t = np.zeros((3, 3, 5))
+    for k in range(3):
+        t[k, k] = c
+    pec_t, lossy_t = conductor_split(t)
+    np.testing.assert_array_equal(pec_t, pec)
+    assert np.all(lossy_t[:, :, 3:] == 0) and lossy_t[1, 1, 1] == 0.2
