@@ -259,16 +259,66 @@ def config_bench(name: str, nf: int, fblock: int) -> dict:
         tc = time.perf_counter() - t0
         ok_s = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in seq)
         ok_c = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in conc)
+        for sw in sws:
+            sw.ctx.close()
+        # the same meshes as one block-diagonal system: one launch per kernel
+        from emerge_amd.family import FamilySweep
+        fam = FamilySweep(probs, [w.waveguide_bcs(q) for q in probs])
+        kfam = dict(kf)
+        fam.solve(freqs[:2], precision="mixed", **kfam)
+        t0 = time.perf_counter()
+        Sf, bound, rf = fam.solve(freqs, precision="mixed", **kfam)
+        tf = time.perf_counter() - t0
+        ok_f = int(np.sum(np.all(bound <= 1e-8, axis=1))) * len(probs)
+        union = {"seconds": round(tf, 3), "converged_points": ok_f, "freq_points_per_s": round(ok_f / tf, 3),
+                 "reduced_dofs": int(fam.sweep.n_red), "mean_iters_per_system": float(rf.iters.mean()),
+                 "max_dS_vs_sequential": float(max(np.abs(Sf[m] - seq[m].S).max() for m in range(len(probs))))}
         out.update(workload="16 jittered WR-90 sections (Kuhn 11x5xnz), DOFs U[20k,120k] seed 0",
                    dofs=[int(q.basis.n_field) for q in probs], points_per_mesh=len(freqs),
                    sequential={"seconds": round(ts, 3), "converged": ok_s, "freq_points_per_s": round(ok_s / ts, 3)},
                    concurrent={"seconds": round(tc, 3), "converged": ok_c, "freq_points_per_s": round(ok_c / tc, 3)},
-                   max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))))
+                   max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))),
+                   one_system=union,
+                   mean_iters_per_system_single=float(np.mean([r.iters.mean() for r in seq])))
     elif name == "farfield":
         out.update(farfield_bench())
+    elif name == "element":
+        out.update(element_bench())
     else:
         raise ValueError(name)
     return out
+
+
+def element_bench(n: int = 60, reps: int = 10) -> dict:
+    """Batched dense element matrices (emg_element_matrices) of the n^3
+    jittered cube, kept on the device: the vector-unit kernel (compile-time
+    sparse tables) against the FP64 matrix-core kernel (dense tables).
+    Algorithmic bytes per tet: 16 connectivity + 32 material + 12,800
+    written (K and M, 400 complex128 each) + 24 per vertex."""
+    from emerge_amd._lib import Context
+    from emerge_amd.workloads import cube_assembly
+    prob = cube_assembly(n, seed=0)
+    m, f = prob.mesh, prob.basis
+    ctx = Context(0)
+    ctx.mesh_set(m.nodes, m.tets, f.tet_to_field, f.n_field)
+    ctx.material_set(prob.eps_r, prob.inv_mu_r)
+    T = m.n_tets
+    algo = T * (16 + 32 + 12800) + 24 * m.n_nodes
+    res = {"workload": f"element matrices of the {n}^3 jittered Kuhn cube ({T} tets), device-resident output",
+           "algorithmic_bytes": algo}
+    for mode, name in ((0, "vector"), (1, "mfma")):
+        for _ in range(2):
+            ctx.call("emg_element_matrices", mode, None, None)
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(reps):
+            ctx.call("emg_element_matrices", mode, None, None)
+        ms = ctx.timer_stop() / reps
+        res[name] = {"ms": round(ms, 4), "Mtets_per_s": round(T / ms / 1e3, 2),
+                     "GB_per_s": round(algo / ms / 1e6, 1), "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
+        print(f"[bench] element {name}: {ms:.3f} ms", file=sys.stderr, flush=True)
+    ctx.close()
+    return res
 
 
 def farfield_bench(n: int = 60, reps: int = 5) -> dict:
@@ -354,7 +404,7 @@ def main() -> int:
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=None,
                     help="GMRES dimension / IDR matvecs per cycle / COCG iterations per cycle")
-    ap.add_argument("--config", choices=["cavity", "patch", "family", "farfield"], default=None,
+    ap.add_argument("--config", choices=["cavity", "patch", "family", "farfield", "element"], default=None,
                     help="run one full-size solve workload instead of the default bench")
     ap.add_argument("--config-points", type=int, default=None, help="frequency points of --config")
     ap.add_argument("--precond", default="aux", choices=["aux", "jacobi"],
@@ -388,7 +438,7 @@ def main() -> int:
     from emerge_amd.workloads import cube_assembly
 
     if args.config:
-        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "farfield": 0}[args.config]
+        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "farfield": 0, "element": 0}[args.config]
         fb = 8 if args.fblock == "auto" else args.fblock
         print(json.dumps(config_bench(args.config, nf, fb if args.config == "patch" else 32)),
               flush=True)

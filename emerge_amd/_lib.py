@@ -66,6 +66,8 @@ SIGNATURES: dict[str, tuple] = {
     "emg_material_set_tensor": (_c_int, [_p, _p, _p]),
     "emg_bc_add_abc2": (_c_int, [_p, _i64, _p, ctypes.POINTER(ctypes.c_int32), _pi64]),
     "emg_csr_set": (_c_int, [_p, _i64, _p, _p, _p]),
+    "emg_csr_aux_set": (_c_int, [_p, _i64, _p, _p, _p, _p, _p, _p]),
+    "emg_element_matrices": (_c_int, [_p, _c_int, _p, _p]),
     "emg_csr_solve": (_c_int, [_p, ctypes.c_int32, _p, _c_int, ctypes.c_double, _c_int, _c_int, _p, _p, _p]),
 }
 

@@ -31,12 +31,39 @@ def isotropic_scalars(tensor: np.ndarray, name: str) -> np.ndarray | None:
     return np.ascontiguousarray(d, dtype=np.complex128)
 
 
+# canonical DOF order (tet vertices ranked by global id): edges (01, 02, 03,
+# 12, 13, 23), faces (012, 013, 023, 123); the reference's local pairs and the
+# vertex each local face misses (E/mesh3d.py:427, 431)
+_CANON_EDGE = {(0, 1): 0, (0, 2): 1, (0, 3): 2, (1, 2): 3, (1, 3): 4, (2, 3): 5}
+_LOCAL_EDGE = ((0, 1), (0, 2), (0, 3), (1, 2), (3, 1), (2, 3))
+_LOCAL_FACE_MISSING = (3, 1, 2, 0)
+
+
+def canonical_dof_map(tets: np.ndarray) -> np.ndarray:
+    """(20, T): the canonical DOF of each local DOF of every tet (the
+    reference's local order is that of tet_to_field)."""
+    w = np.asarray(tets)
+    rk = (w[None, :, :] < w[:, None, :]).sum(axis=1)        # rk[k, t] = rank of local vertex k
+    umap = np.empty((20, w.shape[1]), dtype=np.int64)
+    for e, (a, b) in enumerate(_LOCAL_EDGE):
+        lo, hi = np.minimum(rk[a], rk[b]), np.maximum(rk[a], rk[b])
+        ce = np.zeros(w.shape[1], dtype=np.int64)
+        for (p, q), k in _CANON_EDGE.items():
+            ce[(lo == p) & (hi == q)] = k
+        umap[e], umap[e + 10] = ce, ce + 10
+    for f, miss in enumerate(_LOCAL_FACE_MISSING):
+        cf = 3 - rk[miss]
+        umap[f + 6], umap[f + 16] = cf + 6, cf + 16
+    return umap
+
+
 class MeshOnDevice:
     """A mesh uploaded to one context with its CSR pattern built."""
 
     def __init__(self, ctx: Context, nodes: np.ndarray, tets: np.ndarray, tet_to_field: np.ndarray,
                  n_field: int):
         self.ctx = ctx
+        self.tets = np.asarray(tets)
         ctx.mesh_set(nodes, tets, tet_to_field, n_field)
         self.nnz = ctx.pattern_build()
         self.n_field = int(n_field)
@@ -52,6 +79,23 @@ class MeshOnDevice:
     def assemble(self, eps_r: np.ndarray, inv_mu_r: np.ndarray) -> None:
         self.ctx.material_set(eps_r, inv_mu_r)
         self.ctx.assemble()
+
+    def element_matrices(self, eps_r: np.ndarray, inv_mu_r: np.ndarray, use_mfma: bool = False):
+        """Dense element matrices of every tet (ned2_tet_stiff_mass,
+        ASM/curlcurl.py:180-412, batched on the device): K, M as
+        (T, 20, 20) complex in the reference's local DOF order."""
+        from ._lib import ptr
+        self.ctx.material_set(eps_r, inv_mu_r)
+        T = self.n_tets
+        Kc = np.empty((400, T), dtype=np.complex128)
+        Mc = np.empty((400, T), dtype=np.complex128)
+        self.ctx.call("emg_element_matrices", int(bool(use_mfma)), ptr(Kc), ptr(Mc))
+        umap = canonical_dof_map(self.tets)                    # (20, T)
+        idx = umap[:, None, :] * 20 + umap[None, :, :]         # (20, 20, T)
+        t = np.arange(T)[None, None, :]
+        K = Kc[idx, t].transpose(2, 0, 1)
+        M = Mc[idx, t].transpose(2, 0, 1)
+        return K, M
 
     def csr(self) -> tuple[csr_matrix, csr_matrix]:
         indptr, indices = self.pattern
@@ -80,9 +124,26 @@ class HipAssembler:
             self._ctx = Context(self.device)
         return self._ctx
 
+    @staticmethod
+    def _fingerprint(*arrays) -> tuple:
+        """Content key of the arrays the device mesh is built from, so that a
+        mutated mesh (or a new one at a recycled address) is rebuilt."""
+        try:
+            import xxhash
+            h = xxhash.xxh3_128()
+        except ImportError:  # pragma: no cover
+            import hashlib
+            h = hashlib.blake2b(digest_size=16)
+        shapes = []
+        for a in arrays:
+            a = np.ascontiguousarray(a)
+            shapes.append((a.shape, a.dtype.str))
+            h.update(a.view(np.uint8).reshape(-1))
+        return tuple(shapes), h.hexdigest()
+
     def device_mesh(self, field) -> MeshOnDevice:
         mesh = field.mesh
-        key = (id(mesh), mesh.tets.shape, field.tet_to_field.shape)
+        key = self._fingerprint(mesh.nodes, mesh.tets, field.tet_to_field)
         if self._dev_mesh is None or self._mesh_key != key:
             self._dev_mesh = MeshOnDevice(self._context(), mesh.nodes, mesh.tets, field.tet_to_field,
                                           field.n_field)

@@ -23,6 +23,7 @@ SOURCES = [
     "emg_topology.hip",
     "emg_farfield.hip",
     "emg_fields.hip",
+    "emg_element.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")
@@ -33,6 +34,8 @@ COMMON_FLAGS = [
     # packed FP32 ops off on every HIP translation unit
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # compile-time element tables (emg_element.hip) take many constexpr steps
+    "-fconstexpr-steps=100000000",
 ]
 
 

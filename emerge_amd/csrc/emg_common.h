@@ -180,6 +180,15 @@ struct emg_ctx {
     bool g_have_sym = false;      // symmetric part (A + A^T)/2 of the generic matrix built
     emg::DBuf<double> g_s64;
     emg::DBuf<float> g_s32;
+    // auxiliary gradient space of the generic matrix (emg_csr_aux_set): G and
+    // G^T, and D_g = diag(G^T A G) computed on the device from the matrix
+    // batched element matrices (emg_element_matrices): [400][T] complex
+    emg::DBuf<double> el_K, el_M;
+    int64_t g_aux_n = 0;
+    bool g_aux_fresh = false;     // D_g belongs to the current generic matrix
+    emg::DBuf<int32_t> g_aux_gptr, g_aux_gidx, g_aux_tptr, g_aux_tidx;
+    emg::DBuf<double> g_aux_gval64, g_aux_tval64, g_aux_dg;
+    emg::DBuf<float> g_aux_gval32, g_aux_tval32;
 };
 
 namespace emg {

@@ -1858,6 +1858,45 @@ extern "C" int emg_sweep_escalated(emg_ctx* c, int64_t* n) {
     return EMG_OK;
 }
 
+// D_g[j] = g_j^T A g_j for the columns g_j of G (rows of G^T, sorted):
+// the Galerkin projection of a given matrix onto the gradient space (for a
+// Maxwell operator E - k0^2 B + sum gamma S, E g = 0 and this equals
+// -k0^2 g^T B g + sum gamma g^T S g).  One lane per nodal function; the
+// entries of A's rows are matched against g_j's support by binary search.
+__global__ void k_galerkin_diag(int64_t n_nodal, const int32_t* __restrict__ tptr, const int32_t* __restrict__ tidx,
+                                const double* __restrict__ tval, const int32_t* __restrict__ ptr,
+                                const int32_t* __restrict__ idx, const double2* __restrict__ a,
+                                double2* __restrict__ dg) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_nodal) return;
+    const int b0 = tptr[j], b1 = tptr[j + 1];
+    double sr = 0.0, si = 0.0;
+    for (int q = b0; q < b1; ++q) {
+        const int k = tidx[q];
+        const double gk = tval[q];
+        double ar = 0.0, ai = 0.0;
+        for (int e = ptr[k]; e < ptr[k + 1]; ++e) {
+            const int l = idx[e];
+            int lo = b0, hi = b1 - 1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int v = tidx[mid];
+                if (v < l) lo = mid + 1;
+                else if (v > l) hi = mid - 1;
+                else {
+                    const double2 av = a[e];
+                    ar += av.x * tval[mid];
+                    ai += av.y * tval[mid];
+                    break;
+                }
+            }
+        }
+        sr += gk * ar;
+        si += gk * ai;
+    }
+    dg[j] = make_double2(sr, si);
+}
+
 extern "C" int emg_aux_set(emg_ctx* c, int64_t n_nodal, const int32_t* g_ptr, const int32_t* g_idx,
                            const double* g_val, const int32_t* gt_ptr, const int32_t* gt_idx, const double* gt_val,
                            const double* dvol, int n_surf, const double* dsurf) {
@@ -2115,9 +2154,64 @@ extern "C" int emg_csr_set(emg_ctx* c, int64_t n, const int32_t* indptr, const i
     k_find_diag<<<grid_for(n, 256), 256, 0, s>>>(n, c->g_ptr.p, c->g_idx.p, c->g_diag.p);
     EMG_HIP(c, hipGetLastError());
     EMG_HIP(c, hipStreamSynchronize(s));
+    if (c->g_n != n) c->g_aux_n = 0;   // G belongs to a matrix of another size
     c->g_n = n;
     c->g_have_sym = false;
+    c->g_aux_fresh = false;
     c->g_nnz = nnz;
+    return EMG_OK;
+}
+
+extern "C" int emg_csr_aux_set(emg_ctx* c, int64_t n_nodal, const int32_t* g_ptr, const int32_t* g_idx,
+                               const double* g_val, const int32_t* gt_ptr, const int32_t* gt_idx,
+                               const double* gt_val) {
+    EMG_CHECK_CTX(c);
+    if (n_nodal == 0) {
+        c->g_aux_n = 0;
+        return EMG_OK;
+    }
+    if (c->g_n == 0) return fail(c, EMG_ERR_STATE, "emg_csr_aux_set: no matrix (call emg_csr_set)");
+    if (n_nodal < 0 || n_nodal >= (int64_t(1) << 31) || !g_ptr || !g_idx || !g_val || !gt_ptr || !gt_idx || !gt_val)
+        return fail(c, EMG_ERR_ARG, "emg_csr_aux_set: bad arguments");
+    const int64_t n = c->g_n;
+    const int64_t nnz = g_ptr[n];
+    if (g_ptr[0] != 0 || nnz < 0 || gt_ptr[0] != 0 || gt_ptr[n_nodal] != nnz)
+        return fail(c, EMG_ERR_ARG, "emg_csr_aux_set: G and G^T row pointers disagree");
+    for (int64_t i = 0; i < n; ++i)
+        if (g_ptr[i + 1] < g_ptr[i]) return fail(c, EMG_ERR_ARG, "emg_csr_aux_set: G row pointers not monotone");
+    for (int64_t j = 0; j < n_nodal; ++j) {
+        if (gt_ptr[j + 1] < gt_ptr[j]) return fail(c, EMG_ERR_ARG, "emg_csr_aux_set: G^T row pointers not monotone");
+        for (int64_t e = gt_ptr[j] + 1; e < gt_ptr[j + 1]; ++e)
+            if (gt_idx[e] <= gt_idx[e - 1]) return fail(c, EMG_ERR_ARG, "emg_csr_aux_set: G^T rows not sorted");
+    }
+    for (int64_t e = 0; e < nnz; ++e)
+        if (g_idx[e] < 0 || g_idx[e] >= n_nodal || gt_idx[e] < 0 || gt_idx[e] >= n)
+            return fail(c, EMG_ERR_ARG, "emg_csr_aux_set: column index out of range");
+    hipStream_t s = c->stream;
+    const int64_t nz = std::max<int64_t>(nnz, 1);
+    EMG_HIP(c, c->g_aux_gptr.alloc(n + 1));
+    EMG_HIP(c, c->g_aux_tptr.alloc(n_nodal + 1));
+    EMG_HIP(c, c->g_aux_gidx.alloc(nz));
+    EMG_HIP(c, c->g_aux_tidx.alloc(nz));
+    EMG_HIP(c, c->g_aux_gval64.alloc(nz));
+    EMG_HIP(c, c->g_aux_tval64.alloc(nz));
+    EMG_HIP(c, c->g_aux_gval32.alloc(nz));
+    EMG_HIP(c, c->g_aux_tval32.alloc(nz));
+    EMG_HIP(c, c->g_aux_dg.alloc(2 * n_nodal));
+    EMG_HIP(c, hipMemcpyAsync(c->g_aux_gptr.p, g_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->g_aux_tptr.p, gt_ptr, sizeof(int32_t) * (n_nodal + 1), hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->g_aux_gidx.p, g_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->g_aux_tidx.p, gt_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->g_aux_gval64.p, g_val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->g_aux_tval64.p, gt_val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        k_to_f32<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, c->g_aux_gval64.p, c->g_aux_gval32.p);
+        k_to_f32<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, c->g_aux_tval64.p, c->g_aux_tval32.p);
+    }
+    EMG_HIP(c, hipGetLastError());
+    EMG_HIP(c, hipStreamSynchronize(s));
+    c->g_aux_n = n_nodal;
+    c->g_aux_fresh = false;
     return EMG_OK;
 }
 
@@ -2153,6 +2247,33 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
         op64s.E = reinterpret_cast<const double2*>(c->g_s64.p);
         op32.E = reinterpret_cast<const float2*>(c->g_s32.p);
     }
+    // gradient-space correction (COCG): D_g = diag(G^T A G) of this matrix,
+    // applied through D_g = -k2 dvol with k2 = -1 (the generic matrix has no
+    // separate mass part, so k2 enters nothing else)
+    const bool gaux = c->krylov_method == 2 && c->g_aux_n > 0;
+    if (gaux) {
+        if (!c->g_aux_fresh) {
+            k_galerkin_diag<<<grid_for(c->g_aux_n, 256), 256, 0, s>>>(
+                c->g_aux_n, c->g_aux_tptr.p, c->g_aux_tidx.p, c->g_aux_tval64.p, c->g_ptr.p, c->g_idx.p,
+                reinterpret_cast<const double2*>(c->g_v64.p), reinterpret_cast<double2*>(c->g_aux_dg.p));
+            EMG_HIP(c, hipGetLastError());
+            c->g_aux_fresh = true;
+        }
+        auto wire = [&](auto& o, auto gv, auto tv) {
+            o.aux_n = c->g_aux_n;
+            o.aux_nsurf = 0;
+            o.gptr = c->g_aux_gptr.p;
+            o.gidx = c->g_aux_gidx.p;
+            o.gval = gv;
+            o.tptr = c->g_aux_tptr.p;
+            o.tidx = c->g_aux_tidx.p;
+            o.tval = tv;
+            o.dvol = reinterpret_cast<const double2*>(c->g_aux_dg.p);
+            o.dsurf = nullptr;
+        };
+        wire(op32, c->g_aux_gval32.p, c->g_aux_tval32.p);
+        wire(op64s, c->g_aux_gval64.p, c->g_aux_tval64.p);
+    }
     SolveParams P;
     P.tol = tol;
     P.restart = restart;
@@ -2173,7 +2294,7 @@ extern "C" int emg_csr_solve(emg_ctx* c, int32_t nrhs, const double* b, int prec
         EMG_HIP(c, hipMemcpyAsync(bin.p, b + 2 * done * n, sizeof(double2) * n * F, hipMemcpyHostToDevice, s));
         k_transpose<<<grid_for(n * F, 256), 256, 0, s>>>(n, F, bin.p, b64.p, 0);
         k_zero2<<<grid_for(n * F, 256), 256, 0, s>>>(n * F, x64.p);
-        std::vector<double> k2(F, 0.0);
+        std::vector<double> k2(F, gaux ? -1.0 : 0.0);
         double2 dummy = make_double2(0, 0);
         BatchStats st;
         rc = precision == 1 ? dispatch_batch<float>(c, F, op32, op64, k2.data(), &dummy, 0, b64.p, x64.p, P, st)

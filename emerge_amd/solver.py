@@ -80,6 +80,8 @@ class SolverHIP:
         self.device = device
         self._ctx: Context | None = None
         self._n = -1
+        self._aux = None          # (G, G^T) of the attached mesh, rows = the reduced DOFs
+        self._aux_on_device = False
 
     def __str__(self) -> str:
         return "SolverHIP"
@@ -91,8 +93,44 @@ class SolverHIP:
         self.initialized = True
 
     def duplicate(self) -> "SolverHIP":
-        return self.__class__(self.pre, self.precision, self.tol, self.restart, self.max_cycles, self.device,
-                              self.method, self.inner_rel)
+        out = self.__class__(self.pre, self.precision, self.tol, self.restart, self.max_cycles, self.device,
+                             self.method, self.inner_rel)
+        out._aux = self._aux
+        return out
+
+    def attach(self, mesh, basis, solve_ids=None, pec_tris=None) -> "SolverHIP":
+        """Give the solver the mesh and DOF maps of the systems it will see,
+        so that COCG adds the auxiliary gradient-space correction
+        ``G D_g^-1 G^T`` (D_g = diag(G^T A G) formed on the device per
+        matrix).  SolveRoutine hands A[ix_(solve_ids, solve_ids)]
+        (E/solver.py:1266-1268) with solve_ids ascending, the DOFs left by
+        the PEC elimination (ASM/assembler.py:274-307, 406-407): pass them,
+        or the PEC triangles that define them.  Nodal functions whose
+        gradient touches an eliminated DOF are left out."""
+        from .gradient import discrete_gradient
+        N = basis.n_field
+        if solve_ids is None:
+            t = np.asarray(pec_tris if pec_tris is not None else [], dtype=np.int64)
+            removed = np.concatenate([basis.edge_to_field[:, mesh.tri_to_edge[:, t].reshape(-1)].reshape(-1),
+                                      basis.tri_to_field[:, t].reshape(-1)])
+            keep = np.ones(N, dtype=bool)
+            keep[removed] = False
+            solve_ids = np.nonzero(keep)[0]
+        solve_ids = np.asarray(solve_ids, dtype=np.int64)
+        if np.any(np.diff(solve_ids) <= 0):
+            raise ValueError("solve_ids must be ascending (the order SolveRoutine uses)")
+        G = discrete_gradient(mesh, basis).tocsr()
+        dropped = np.ones(N, dtype=bool)
+        dropped[solve_ids] = False
+        touches = np.asarray(abs(G[dropped]).sum(axis=0)).ravel() > 0
+        cols = np.nonzero(~touches)[0]
+        Gr = G[solve_ids][:, cols].tocsr()
+        Gr.sort_indices()
+        Gt = Gr.T.tocsr()
+        Gt.sort_indices()
+        self._aux = (Gr, Gt)
+        self._aux_on_device = False
+        return self
 
     def set_options(self, pivoting_threshold: float | None = None) -> None:
         """Pivoting thresholds do not apply to a Krylov solver."""
@@ -112,6 +150,18 @@ class SolverHIP:
             v = np.ascontiguousarray(Ac.data, dtype=np.complex128)
             self._ctx.call("emg_csr_set", n, ptr(ip), ptr(ix), ptr(v))
             self._n = n
+            if self._aux is not None and not self._aux_on_device:
+                Gr, Gt = self._aux
+                if Gr.shape[0] != n:
+                    raise SimulationError(f"SolverHIP: attached mesh has {Gr.shape[0]} reduced DOFs, "
+                                          f"the system {n}")
+                gp, gi = np.ascontiguousarray(Gr.indptr, np.int32), np.ascontiguousarray(Gr.indices, np.int32)
+                gv = np.ascontiguousarray(Gr.data, np.float64)
+                tp, ti = np.ascontiguousarray(Gt.indptr, np.int32), np.ascontiguousarray(Gt.indices, np.int32)
+                tv = np.ascontiguousarray(Gt.data, np.float64)
+                self._ctx.call("emg_csr_aux_set", Gr.shape[1], ptr(gp), ptr(gi), ptr(gv), ptr(tp), ptr(ti),
+                               ptr(tv))
+                self._aux_on_device = True
         bb = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(-1))
         x = np.empty(n, dtype=np.complex128)
         rel = np.empty(1)
@@ -124,5 +174,6 @@ class SolverHIP:
         except EmgError as e:
             raise SimulationError(str(e)) from e
         aux = {"precision": self.precision, "method": self.method, "iterations": str(int(it[0])), "relres": f"{rel[0]:.3e}",
-               "tolerance": str(self.tol)}
+               "tolerance": str(self.tol),
+               "gradient_space": str(self._aux is not None and self.method == "cocg")}
         return x, SolveReport(solver=str(self), exit_code=0, aux=aux, simtime=time.time() - t0)

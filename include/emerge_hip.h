@@ -287,11 +287,33 @@ int emg_aux_set(emg_ctx* ctx, int64_t n_nodal, const int32_t* g_ptr, const int32
                 const int32_t* gt_ptr, const int32_t* gt_idx, const double* gt_val, const double* dvol,
                 int n_surf, const double* dsurf);
 
+/* Dense element matrices of every tet of the context's mesh (isotropic
+ * materials of emg_material_set): the 20x20 curl-curl K and mass M of
+ * ned2_tet_stiff_mass (ASM/curlcurl.py:180-412), batched.  Layout
+ * K[e][t], M[e][t] complex128 with e = 20 u + v over the canonical DOFs u, v
+ * (tet vertices ranked by global id; DOF order edges (01,02,03,12,13,23),
+ * faces (012,013,023,123), then the second kind of each); K and M may be
+ * null to keep the result on the device.  use_mfma = 1 contracts the tets'
+ * Gram vectors with the dense tables on FP64 matrix cores, 0 with the
+ * compile-time sparse tables on the vector units. */
+int emg_element_matrices(emg_ctx* ctx, int use_mfma, double* K, double* M);
+
 /* Generic complex CSR system for the Solver plug-in (replaces
  * Solver.solve(A, b, ...), E/solver.py:365-367, next to SuperLU / PARDISO /
  * cuDSS, E/solver.py:535-773): vals complex128 interleaved. */
 int emg_csr_set(emg_ctx* ctx, int64_t n, const int32_t* indptr, const int32_t* indices,
                 const double* vals);
+/* Auxiliary gradient space for the generic matrix (COCG): G (n x n_nodal,
+ * real CSR g_*, rows = the matrix rows) and its transpose (gt_*, each row
+ * sorted), the discrete gradient of the nodal functions whose gradients lie
+ * in the matrix's DOF space.  D_g = diag(G^T A G) is formed on the device
+ * from the current matrix before a solve (for E - k0^2 B + sum gamma S it
+ * equals the closed form emg_aux_set takes, E g = 0).  Lets the Solver
+ * plug-in precondition Maxwell systems as the batched sweep does; no
+ * counterpart in the reference (its iterative path uses ILU, E/solver.py:
+ * 437-527).  n_nodal = 0 turns it off; a matrix of another size drops it. */
+int emg_csr_aux_set(emg_ctx* ctx, int64_t n_nodal, const int32_t* g_ptr, const int32_t* g_idx, const double* g_val,
+                    const int32_t* gt_ptr, const int32_t* gt_idx, const double* gt_val);
 /* nrhs right-hand sides b (complex [nrhs][n]) -> x (complex [nrhs][n]).
  * Returns EMG_ERR_CONVERGE (x still written) if a system misses tol. */
 int emg_csr_solve(emg_ctx* ctx, int32_t nrhs, const double* b, int precision, double tol,

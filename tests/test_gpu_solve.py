@@ -358,3 +358,45 @@ def test_auto_precision_policy(sweep):
     assert np.abs(res.S[0] - o["S"]).max() <= 1e-6
     with pytest.raises(ValueError):
         sweep.solve([F0], precision="fp16")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solver_plugin_with_attached_mesh(precision):
+    """SolverHIP with the mesh attached (gradient-space correction, D_g =
+    diag(G^T A G) formed on the device) on the systems SolveRoutine hands it
+    (A[ix_(ids, ids)], E/solver.py:1266-1268): the lossy lumped-port cavity
+    and the WR-90 guide against SuperLU, and fewer COCG iterations than
+    without the mesh."""
+    from tests import cavity_case
+    cases = []
+    o = cavity_case.oracle_solution(6e9)
+    cc_ = cavity_case.case()
+    cases.append(("cavity", cc_["mesh"], cc_["basis"], o, 1))
+    w = wg_case.oracle_solution(F0)
+    cw = wg_case.case()
+    cases.append(("wr90", cw["mesh"], cw["basis"], w, 1))
+    for name, mesh, basis, oc, port in cases:
+        ids = oc["ids"]
+        A = oc["K"].tocsc()[np.ix_(ids, ids)]
+        b = (oc["b"] + oc["pv"][port])[ids]
+        x_ref = oc["fields"][port][ids]
+        from emerge_amd.solver import SimulationError
+        plain = SolverHIP("", precision=precision, tol=1e-9, method="cocg", restart=20000, max_cycles=5)
+        try:
+            it0 = int(plain.solve(A, b, None, False, 0)[1].aux["iterations"])
+        except SimulationError:   # Jacobi alone may not get there
+            it0 = None
+        att = SolverHIP("", precision=precision, tol=1e-9, method="cocg", restart=40000,
+                        max_cycles=60).attach(mesh, basis, solve_ids=ids)
+        x1, rep1 = att.solve(A, b, None, False, 0)
+        it1 = int(rep1.aux["iterations"])
+        rr = np.linalg.norm(b - A @ x1) / np.linalg.norm(b)
+        dx = np.linalg.norm(x1 - x_ref) / np.linalg.norm(x_ref)
+        print(f"{name} {precision}: iterations {it0} -> {it1} with the mesh, relres {rr:.1e}, |dx| {dx:.1e}",
+              flush=True)
+        assert rr <= 1e-9 and dx <= 1e-6
+        assert rep1.aux["gradient_space"] == "True"
+        assert it0 is None or it1 < it0
+        # a second right-hand side on the same matrix reuses the device state
+        x2, rep2 = att.solve(A, 2 * b, None, True, 1)
+        assert np.linalg.norm(x2 - 2 * x1) <= 1e-6 * np.linalg.norm(x1)
