@@ -12,6 +12,7 @@ runs on a single GPU.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -497,12 +498,16 @@ def main() -> int:
     T, V, nnz = m.n_tets, m.n_nodes, dm.nnz
     algo = assembly_algorithmic_bytes(T, V, nnz)
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
-    # this workload (copy-calibrated); only for the 60^3 cube it was taken on
+    # this workload (copy-calibrated); reported only for the 60^3 cube it was
+    # taken on and only while the kernel source is the one it was measured on
     traffic = None
     tf = os.path.join(ROOT, "profiles", "assembly_cube60_hbm_traffic.json")
     if args.n == 60 and os.path.exists(tf):
         with open(tf) as fh:
-            traffic = int(json.load(fh)["traffic_bytes"])
+            rec = json.load(fh)
+        with open(os.path.join(ROOT, rec["source"]), "rb") as fh:
+            same = hashlib.sha256(fh.read()).hexdigest() == rec["source_sha256"]
+        traffic = int(rec["traffic_bytes"]) if same else None
     achieved = algo / (ms * 1e-3) / 1e9
     value = T / (ms * 1e-3) / 1e6
 

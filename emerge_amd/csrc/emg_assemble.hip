@@ -15,6 +15,8 @@
 #include "emg_common.h"
 #include "emg_ned2.h"
 
+#include <type_traits>
+
 namespace {
 
 using namespace emg;
@@ -24,17 +26,33 @@ using namespace emg;
 // [20 l, 20 l + 20), finishes the slots that end in its range and passes the
 // partial sum of a slot continuing from lane l-1 to that slot's first lane
 // (LDS carry), which adds the carries in lane order.  Every lane does the
-// same work whatever the slot sizes.
-__global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
+// same work whatever the slot sizes.  The finished slot values are held in
+// registers until every lane has read the element rows, then written into
+// the same LDS (the chunk's slots are one contiguous CSR range of at most
+// kAsmMaxSlots entries) and copied out with coalesced 16-byte stores: the
+// wavefront writes 1 KB of consecutive values per store instead of 64
+// scattered 16-byte pieces.
+//
+// kRealK: every tet's 1/mu_r is real (the common case), so the curl-curl
+// values are real: that pass stages its rows already multiplied by the
+// lane's real factor and sums one 8-byte LDS read per code (the imaginary
+// parts are exact zeros).
+template <bool kRealK>
+__global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_assemble_rows(
     const Chunk* __restrict__ chunks, int64_t n_chunks, const int32_t* __restrict__ inc_tet,
     const uint8_t* __restrict__ inc_u, const int4* __restrict__ inc_vid, const uint16_t* __restrict__ grp_slot,
     const uint16_t* __restrict__ contrib, const double* __restrict__ xyz, int64_t V,
     const double2* __restrict__ eps, const double2* __restrict__ inu, double2* __restrict__ Ev,
     double2* __restrict__ Bv) {
-    __shared__ double stage[20][kAsmThreads];  // element row of every lane
+    // element rows of every lane (20 x kAsmThreads doubles, 40 KB) while the
+    // sums run; the chunk's finished slot values (kAsmMaxSlots double2,
+    // 64 KB) afterwards
+    __shared__ double2 obuf[kAsmMaxSlots];
     __shared__ double2 fac[kAsmThreads];       // complex row factor of every lane
     __shared__ double2 carry[kAsmThreads];     // partial sum of a slot continued from the previous lane
     __shared__ int closed[kAsmThreads];        // whether that slot ends in this lane's range
+    static_assert(20 * kAsmThreads * sizeof(double) <= kAsmMaxSlots * sizeof(double2), "stage fits obuf");
+    double* stage = reinterpret_cast<double*>(obuf);  // stage[c * kAsmThreads + lane]
 
     // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so
     // give each XCD a contiguous range of chunks (neighbouring rows share
@@ -94,10 +112,14 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
     const bool tail_open = active && (cw[9] & (uint32_t(kCodeLast) << 16)) == 0;
 
     // pass 0: curl-curl -> Ev; pass 1: mass -> Bv
-#pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {
+    auto run_pass = [&](auto pass_tag) __attribute__((always_inline)) {
+        constexpr int pass = decltype(pass_tag)::value;
+        constexpr bool real = pass == 0 && kRealK;
         if (active) {
             // wave-uniform loop over the distinct canonical rows of this wavefront
+            double d[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) d[k] = real ? G.d[k] * kf.x : G.d[k];
             bool pending = true;
             while (true) {
                 const uint64_t m = __ballot(pending);
@@ -111,67 +133,133 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(
                     // compiler from hoisting all 20 switch cases out of this
                     // loop.
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) asm volatile("" : "+v"(G.d[k]));
-                    if (pass == 0) {
+                    for (int k = 0; k < 6; ++k) asm volatile("" : "+v"(d[k]));
+                    if constexpr (pass == 0) {
 #pragma unroll
                         for (int k = 0; k < 21; ++k) asm volatile("" : "+v"(G.h[k]));
-                        ned2::stiff_row(u0, G.h, G.d, &stage[0][tid], kAsmThreads);
+                        ned2::stiff_row(u0, G.h, d, stage + tid, kAsmThreads);
                     } else {
 #pragma unroll
                         for (int k = 0; k < 10; ++k) asm volatile("" : "+v"(G.g[k]));
-                        ned2::mass_row(u0, G.g, G.d, &stage[0][tid], kAsmThreads);
+                        ned2::mass_row(u0, G.g, d, stage + tid, kAsmThreads);
                     }
                     pending = false;
                 }
             }
-            fac[tid] = make_double2(pass == 0 ? kf.x : mf.x, pass == 0 ? kf.y : mf.y);
+            if constexpr (!real) fac[tid] = pass == 0 ? kf : mf;
         }
         __syncthreads();
-        double2* out = (pass == 0 ? Ev : Bv) + ch.slot0;
+        // Segmented sums, branch-free.  res[r] is the running sum of code r's
+        // slot within this lane's range (the slot's value where code r is
+        // its last).  A lane whose first codes continue the previous lane's
+        // slot (cont) sums that prefix into (cre, cim) and hands it over
+        // through carry[].  The codes are marked as rewritten here so that
+        // the LDS addresses derived from them are formed in this phase, not
+        // hoisted out of the pass loop and held through the element rows.
+        // The reads go in batches of 5 codes, so that each batch's LDS
+        // round trip is paid once.
+        uint32_t cv[10];
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            cv[r] = cw[r];
+            asm volatile("" : "+v"(cv[r]));
+        }
+        double2 res[20];
         int slot = slot_first;
-        bool first = true;
+        bool pre = cont;  // still inside the slot continued from the previous lane
         double re = 0.0, im = 0.0;
         if (active) {
+            double cre = 0.0, cim = 0.0;
 #pragma unroll
-            for (int r = 0; r < 20; ++r) {
-                const uint32_t code = (cw[r >> 1] >> (16 * (r & 1))) & 0xffffu;
-                const int ln = (code >> 5) & 0xff;
-                const double x = stage[code & 31][ln];
-                const double2 f = fac[ln];
-                re = __builtin_fma(f.x, x, re);
-                im = __builtin_fma(f.y, x, im);
-                if (code & kCodeLast) {
-                    if (first && cont) {
-                        carry[tid] = make_double2(re, im);
-                        closed[tid] = 1;
+            for (int b0 = 0; b0 < 20; b0 += 5) {
+                double xs[5];
+                double2 fs[5];
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    const int r = b0 + j;
+                    const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                    const int ln = (code >> 5) & 0xff;
+                    xs[j] = stage[(code & 31) * kAsmThreads + ln];
+                    if constexpr (!real) fs[j] = fac[ln];
+                }
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    const int r = b0 + j;
+                    const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                    if constexpr (real) {
+                        re += xs[j];
                     } else {
-                        out[slot] = make_double2(re, im);
+                        re = __builtin_fma(fs[j].x, xs[j], re);
+                        im = __builtin_fma(fs[j].y, xs[j], im);
                     }
-                    ++slot;
-                    re = 0.0;
-                    im = 0.0;
-                    first = false;
+                    res[r] = make_double2(re, im);
+                    const bool last = (code & kCodeLast) != 0;
+                    cre = pre ? re : cre;
+                    if constexpr (!real) cim = pre ? im : cim;
+                    pre = pre && !last;
+                    slot += last ? 1 : 0;
+                    re = last ? 0.0 : re;
+                    if constexpr (!real) im = last ? 0.0 : im;
                 }
             }
-            if (tail_open && first && cont) {  // the whole range lies inside one slot
-                carry[tid] = make_double2(re, im);
-                closed[tid] = 0;
+            if (cont) {
+                carry[tid] = make_double2(cre, cim);
+                closed[tid] = pre ? 0 : 1;  // 0: the whole range lies inside that slot
             }
         }
-        __syncthreads();
+        __syncthreads();  // element rows read, carries written: obuf may now take the slot values
+        // Every code writes its running sum to its slot (the slot's last code
+        // writes last, in program order, so the final value stays); codes of
+        // the continued prefix write to this lane's own fac entry, which is
+        // no longer read in this pass.
+        if (active) {
+            int s = slot_first;
+            bool p = cont;
+#pragma unroll
+            for (int r = 0; r < 20; ++r) {
+                const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                const bool last = (code & kCodeLast) != 0;
+                double2* dst = p ? &fac[tid] : &obuf[s];
+                *dst = res[r];
+                p = p && !last;
+                s += last ? 1 : 0;
+            }
+        }
         // a slot that starts in this lane's range and runs past it: add the
         // following lanes' partial sums in order (the chunk's last code
         // always ends a slot, so the walk stops inside the chunk)
-        if (tail_open && !(first && cont)) {
+        if (tail_open && !pre) {
             for (int q = tid + 1; q < ch.ninc; ++q) {
                 const double2 cq = carry[q];
                 re += cq.x;
                 im += cq.y;
                 if (closed[q]) break;
             }
-            out[slot] = make_double2(re, im);
+            obuf[slot] = make_double2(re, im);
         }
-    }
+        __syncthreads();
+        // copy-out: all (at most kAsmMaxSlots / kAsmThreads = 16) LDS reads
+        // of a lane first, then its coalesced stores
+        double2* out = (pass == 0 ? Ev : Bv) + ch.slot0;
+        {
+            constexpr int kPer = kAsmMaxSlots / kAsmThreads;
+            double vr[kPer], vi[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const double2 t = obuf[j * kAsmThreads + tid];  // < kAsmMaxSlots: in bounds
+                vr[j] = t.x;
+                vi[j] = t.y;
+            }
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int i = j * kAsmThreads + tid;
+                if (i < ch.nslot) out[i] = make_double2(vr[j], vi[j]);
+            }
+        }
+        __syncthreads();  // obuf is the next pass's stage
+    };
+    run_pass(std::integral_constant<int, 0>{});
+    run_pass(std::integral_constant<int, 1>{});
 }
 
 
@@ -390,6 +478,7 @@ __global__ void k_matinv(int64_t T, const double2* __restrict__ in, double2* __r
 #pragma unroll
     for (int k = 0; k < 9; ++k) out[int64_t(k) * T + t] = R[k / 3][k % 3];
 }
+
 }  // namespace
 
 extern "C" int emg_assemble(emg_ctx* c) {
@@ -409,7 +498,8 @@ extern "C" int emg_assemble(emg_ctx* c) {
         c->have_values = true;
         return EMG_OK;
     }
-    k_assemble_rows<<<unsigned(per * 8), kAsmThreads, 0, c->stream>>>(
+    auto kern = c->inu_real ? k_assemble_rows<true> : k_assemble_rows<false>;
+    kern<<<unsigned(per * 8), kAsmThreads, 0, c->stream>>>(
         c->chunks.p, c->n_chunks, c->inc_tet.p, c->inc_u.p, reinterpret_cast<const int4*>(c->inc_vid.p),
         c->grp_slot.p, c->contrib.p, c->xyz.p, c->n_nodes, reinterpret_cast<const double2*>(c->eps.p),
         reinterpret_cast<const double2*>(c->inu.p),

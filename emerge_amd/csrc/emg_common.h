@@ -90,6 +90,7 @@ struct emg_ctx {
     bool have_material = false;
     emg::DBuf<double> eps;        // 2 x T (re, im)
     emg::DBuf<double> inu;        // 2 x T
+    bool inu_real = false;        // every 1/mu_r has a zero imaginary part
     bool have_tensor = false;     // general tensors set (emg_material_set_tensor)
     emg::DBuf<double> eps9;       // complex (3,3,T): eps_r per component, t fastest
     emg::DBuf<double> imu9;       // complex (3,3,T): mu_r^-1

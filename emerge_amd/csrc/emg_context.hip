@@ -109,6 +109,10 @@ int emg_material_set(emg_ctx* c, const double* eps, const double* inu) {
     EMG_HIP(c, hipMemcpyAsync(c->eps.p, eps, sizeof(double) * 2 * c->n_tets, hipMemcpyHostToDevice, c->stream));
     EMG_HIP(c, hipMemcpyAsync(c->inu.p, inu, sizeof(double) * 2 * c->n_tets, hipMemcpyHostToDevice, c->stream));
     EMG_HIP(c, hipStreamSynchronize(c->stream));
+    // all 1/mu_r real: the curl-curl pass of the assembly sums real values
+    bool real = true;
+    for (int64_t t = 0; t < c->n_tets && real; ++t) real = inu[2 * t + 1] == 0.0;
+    c->inu_real = real;
     c->have_material = true;
     c->have_tensor = false;
     return EMG_OK;
