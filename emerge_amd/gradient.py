@@ -28,10 +28,11 @@ import numpy as np
 import scipy.sparse as sp
 
 
-def discrete_gradient(mesh, basis) -> sp.csr_matrix:
-    """(N, V + nE) real CSR: column a is grad lam_a, column V + e is
-    grad(lam_x lam_y) of edge e = (x, y), in the global DOF numbering."""
-    V, nE, nF = mesh.n_nodes, mesh.n_edges, mesh.n_tris
+def _whitney_and_bubbles(mesh, basis):
+    """(Wexp, Ge), both (N, nE) real CSR in the global DOF numbering:
+    column e of Wexp is the lowest-order Whitney function W_xy of edge
+    e = (x, y), x < y, and column e of Ge is grad(lam_x lam_y)."""
+    nE, nF = mesh.n_edges, mesh.n_tris
     N = basis.n_field
     L = mesh.edge_lengths
     e1 = basis.edge_to_field[0]
@@ -41,9 +42,8 @@ def discrete_gradient(mesh, basis) -> sp.csr_matrix:
     t2e = mesh.tri_to_edge              # (pq, qr, pr) of the sorted triangle
     s1 = L[t2e[2]]                      # |pr|
     s2 = L[t2e[0]]                      # |pq|
-    tri = np.arange(nF)
 
-    # Wexp (N, nE): W_xy of every edge in the Nedelec-2 basis
+    # W_xy = lam_x W_xy + lam_y W_xy + sum over faces xyc of lam_c W_xy
     rows = [e1, e2]
     cols = [np.arange(nE), np.arange(nE)]
     vals = [-1.0 / L, -1.0 / L]
@@ -66,16 +66,52 @@ def discrete_gradient(mesh, basis) -> sp.csr_matrix:
         bcols += [e, e]
         bvals += [b1, b2]
     Wexp = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, nE))
+    Ge = sp.csr_matrix((np.concatenate(bvals), (np.concatenate(brows), np.concatenate(bcols))), shape=(N, nE))
+    return Wexp, Ge
+
+
+def discrete_gradient(mesh, basis) -> sp.csr_matrix:
+    """(N, V + nE) real CSR: column a is grad lam_a, column V + e is
+    grad(lam_x lam_y) of edge e = (x, y), in the global DOF numbering."""
+    V, nE = mesh.n_nodes, mesh.n_edges
+    Wexp, Ge = _whitney_and_bubbles(mesh, basis)
     # grad lam_a = sum over edges (x, y): +W_xy where a = y, -W_xy where a = x
     inc = sp.csr_matrix((np.concatenate([np.ones(nE), -np.ones(nE)]),
                          (np.concatenate([np.arange(nE), np.arange(nE)]),
                           np.concatenate([mesh.edges[1], mesh.edges[0]]))), shape=(nE, V))
     Gv = (Wexp @ inc).tocsr()
-    Ge = sp.csr_matrix((np.concatenate(bvals), (np.concatenate(brows), np.concatenate(bcols))), shape=(N, nE))
     G = sp.hstack([Gv, Ge], format="csr")
     G.sum_duplicates()
     G.eliminate_zeros()
     return G
+
+
+def vector_interpolation(mesh, basis) -> sp.csr_matrix:
+    """(N, 3 V) real CSR: column d V + a holds the Nedelec-2 coefficients of
+    the continuous piecewise-linear vector field lam_a e_d (the second-order
+    space contains every linear field, so the interpolation is exact).
+
+    With e_d = sum_k x_k,d grad lam_k and sum_k grad lam_k = 0 on every tet,
+    lam_a e_d = sum_{edges ak} (x_k - x_a)_d lam_a grad lam_k, and
+    lam_a grad lam_k = (W_ak + grad(lam_a lam_k)) / 2, so for the edge
+    e = (x, y), x < y, with t = x_y - x_x: column (a = x) gets
+    t_d (W_xy + grad(lam_x lam_y)) / 2 and column (a = y) gets
+    t_d (W_xy - grad(lam_x lam_y)) / 2."""
+    V, nE = mesh.n_nodes, mesh.n_edges
+    Wexp, Ge = _whitney_and_bubbles(mesh, basis)
+    x, y = mesh.edges[0], mesh.edges[1]
+    t = mesh.nodes[:, y] - mesh.nodes[:, x]
+    ar = np.concatenate([np.arange(nE), np.arange(nE)])
+    cols = np.concatenate([x, y])
+    blocks = []
+    for d in range(3):
+        both = sp.csr_matrix((np.concatenate([t[d], t[d]]), (ar, cols)), shape=(nE, V))
+        signed = sp.csr_matrix((np.concatenate([t[d], -t[d]]), (ar, cols)), shape=(nE, V))
+        blocks.append(0.5 * (Wexp @ both) + 0.5 * (Ge @ signed))
+    P = sp.hstack(blocks, format="csr")
+    P.sum_duplicates()
+    P.eliminate_zeros()
+    return P
 
 
 def _tet_gradients(nodes: np.ndarray, tets: np.ndarray):

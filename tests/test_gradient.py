@@ -131,3 +131,60 @@ def test_nodal_surface_diagonal_is_gSg(mesh):
     ref = np.asarray(G.multiply(S @ G).sum(axis=0)).ravel()
     got = gr.nodal_surface_diagonal(m, tri_ids)
     assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def _pointwise_vector_error(m, f, P, seed):
+    """max |sum_i (P c)_i N_i - sum_a c_{a,d} lam_a e_d| over interior points
+    of three tets, relative to the field's size."""
+    rng = np.random.default_rng(seed)
+    V = m.n_nodes
+    c = rng.standard_normal(3 * V)
+    u = P @ c
+    worst = 0.0
+    for t in (0, 5, m.n_tets - 1):
+        vid = m.tets[:, t]
+        verts = m.nodes[:, vid]
+        pts = verts @ rng.dirichlet(np.ones(4), size=5).T
+        edges_g = m.edges[:, m.tet_to_edge[:, t]]
+        tris_g = m.tris[:, m.tet_to_tri[:, t]]
+        lem = cc.local_mapping(vid[:, None], edges_g[:, :, None])[:, :, 0]
+        ltm = cc.local_mapping(vid[:, None], tris_g[:, :, None])[:, :, 0]
+        N = qd.basis_values(pts, verts[0], verts[1], verts[2], lem, ltm)
+        field = np.einsum("i,icp->cp", u[f.tet_to_field[:, t]], N)
+        lam, _ = _bary(verts, pts)
+        ref = np.zeros_like(field)
+        for d in range(3):
+            for a in range(4):
+                ref[d] += c[d * V + vid[a]] * lam[a]
+        worst = max(worst, np.abs(field - ref).max() / np.abs(ref).max())
+    return worst
+
+
+def test_vector_interpolation_pointwise(mesh):
+    """sum_i (P c)_i N_i equals the piecewise-linear field sum_a c_a lam_a
+    (the second-order basis holds every linear field exactly); one
+    coefficient of P off by 10% does not."""
+    m, f = mesh
+    P = gr.vector_interpolation(m, f)
+    assert P.shape == (f.n_field, 3 * m.n_nodes)
+    assert _pointwise_vector_error(m, f, P, 3) <= 1e-10
+    Pb = P.tolil()
+    row = f.tet_to_field[12, 0]  # a face DOF of the first sampled tet
+    j = Pb.rows[row][int(np.argmax(np.abs(Pb.data[row])))]
+    Pb[row, j] = 1.1 * Pb[row, j]
+    assert _pointwise_vector_error(m, f, Pb.tocsr(), 3) > 1e-3
+
+
+def test_vector_interpolation_constants_are_coordinate_gradients(mesh):
+    """The constant field e_d = sum_a lam_a e_d is grad x_d: P 1_d = G x_d."""
+    m, f = mesh
+    P = gr.vector_interpolation(m, f)
+    G = gr.discrete_gradient(m, f)
+    V = m.n_nodes
+    for d in range(3):
+        ones = np.zeros(3 * V)
+        ones[d * V:(d + 1) * V] = 1.0
+        x = np.zeros(G.shape[1])
+        x[:V] = m.nodes[d]
+        a, b = P @ ones, G @ x
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
