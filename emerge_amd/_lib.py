@@ -57,6 +57,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_port_norms_get": (_c_int, [_p, _p]),
     "emg_solver_config": (_c_int, [_p, _c_int, _c_int, ctypes.c_double]),
     "emg_aux_set": (_c_int, [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _c_int, _p]),
+    "emg_aux_set_terms": (_c_int, [_p, _p, _p]),
     "emg_sweep_escalated": (_c_int, [_p, _p]),
     "emg_solver_set_escalation": (_c_int, [_p, _c_int]),
     "emg_solver_set_safe32": (_c_int, [_p, _c_int]),

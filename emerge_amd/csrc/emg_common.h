@@ -172,6 +172,9 @@ struct emg_ctx {
     emg::DBuf<float> aux_gval32, aux_tval32;
     emg::DBuf<double> aux_dvol;   // 2 x aux_n
     emg::DBuf<double> aux_dsurf;  // aux_nsurf x aux_n
+    bool aux_have_terms = false;  // emg_aux_set_terms: frequency-independent diagonal part and weights
+    emg::DBuf<double> aux_dconst; // 2 x aux_n (complex)
+    emg::DBuf<double> aux_wt;     // aux_n
 
     // generic CSR system handed in by the Solver plug-in
     int64_t g_n = 0, g_nnz = 0;

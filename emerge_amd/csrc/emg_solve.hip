@@ -40,9 +40,10 @@ struct Op {
     const int32_t* sbc = nullptr;
     const R* sval = nullptr;
     const int32_t* diag = nullptr; // slot of the diagonal per row (-1 if none)
-    // auxiliary gradient space of the COCG preconditioner D^-1 + G D_g^-1 G^T
-    // (aux_n nodal functions; 0 = Jacobi only): G (n x aux_n) and G^T as CSR,
-    // D_g = -k2 dvol + sum_s gam[s] dsurf[s] per nodal function
+    // auxiliary space of the COCG preconditioner D^-1 + G W D_g^-1 G^T
+    // (aux_n functions; 0 = Jacobi only): G (n x aux_n) and G^T as CSR,
+    // D_g = dconst - k2 dvol + sum_s gam[s] dsurf[s] per function, W the
+    // per-function weights (dconst 0 and W 1 when not set)
     int64_t aux_n = 0;
     int aux_nsurf = 0;
     const int32_t* gptr = nullptr;
@@ -53,6 +54,8 @@ struct Op {
     const R* tval = nullptr;
     const double2* dvol = nullptr;
     const double* dsurf = nullptr;  // [aux_nsurf][aux_n]
+    const double2* dconst = nullptr; // [aux_n] complex, or null
+    const double* wt = nullptr;     // [aux_n] or null
     // complex64 COCG only: the stiffness values in FP64 when they are real
     // (real mu_r); the SpMV then accumulates in FP64
     const double* Er64 = nullptr;
@@ -1308,13 +1311,18 @@ __global__ void k_aux_dg(Op<R> op, const double* __restrict__ k2, const double2*
     const int f = int(k % F);
     const double2 dv = op.dvol[j];
     double dr = -k2[f] * dv.x, di = -k2[f] * dv.y;
+    if (op.dconst) {
+        dr += op.dconst[j].x;
+        di += op.dconst[j].y;
+    }
     for (int sb = 0; sb < op.aux_nsurf; ++sb) {
         const double w = op.dsurf[int64_t(sb) * op.aux_n + j];
         dr += gam[sb * F + f].x * w;
         di += gam[sb * F + f].y * w;
     }
     const double m2 = dr * dr + di * di;
-    dginv[k] = m2 > 0 ? from_d<A>(make_double2(dr / m2, -di / m2)) : cmk<A>(A(0), A(0));
+    const double wj = op.wt ? op.wt[j] : 1.0;
+    dginv[k] = m2 > 0 ? from_d<A>(make_double2(wj * dr / m2, -wj * di / m2)) : cmk<A>(A(0), A(0));
 }
 
 // t = D_g^-1 o (G^T r), layout [nodal][f].  A nodal row of G^T (a vertex
@@ -1950,6 +1958,36 @@ extern "C" int emg_aux_set(emg_ctx* c, int64_t n_nodal, const int32_t* g_ptr, co
     EMG_HIP(c, hipStreamSynchronize(s));
     c->aux_n = n_nodal;
     c->aux_nsurf = n_surf;
+    c->aux_have_terms = false;
+    return EMG_OK;
+}
+
+extern "C" int emg_aux_set_terms(emg_ctx* c, const double* dconst, const double* weight) {
+    EMG_CHECK_CTX(c);
+    if (c->aux_n == 0) return fail(c, EMG_ERR_STATE, "emg_aux_set_terms: no auxiliary space (emg_aux_set)");
+    if (!dconst && !weight) {
+        c->aux_have_terms = false;
+        return EMG_OK;
+    }
+    const int64_t na = c->aux_n;
+    std::vector<double> dc(2 * na, 0.0), wt(na, 1.0);
+    for (int64_t j = 0; j < na; ++j) {
+        if (dconst) {
+            dc[2 * j] = dconst[2 * j];
+            dc[2 * j + 1] = dconst[2 * j + 1];
+        }
+        if (weight) wt[j] = weight[j];
+        if (!std::isfinite(dc[2 * j]) || !std::isfinite(dc[2 * j + 1]) || !std::isfinite(wt[j]) || wt[j] < 0)
+            return fail(c, EMG_ERR_ARG, "emg_aux_set_terms: non-finite term or negative weight at %lld",
+                        (long long)j);
+    }
+    hipStream_t s = c->stream;
+    EMG_HIP(c, c->aux_dconst.alloc(2 * na));
+    EMG_HIP(c, c->aux_wt.alloc(na));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_dconst.p, dc.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipMemcpyAsync(c->aux_wt.p, wt.data(), sizeof(double) * na, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, hipStreamSynchronize(s));
+    c->aux_have_terms = true;
     return EMG_OK;
 }
 
@@ -2048,6 +2086,8 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
             o.tval = tv;
             o.dvol = reinterpret_cast<const double2*>(c->aux_dvol.p);
             o.dsurf = c->aux_dsurf.p;
+            o.dconst = c->aux_have_terms ? reinterpret_cast<const double2*>(c->aux_dconst.p) : nullptr;
+            o.wt = c->aux_have_terms ? c->aux_wt.p : nullptr;
         };
         wire(op32, c->aux_gval32.p, c->aux_tval32.p);
         wire(op64s, c->aux_gval64.p, c->aux_tval64.p);

@@ -177,3 +177,52 @@ def nodal_surface_diagonal(mesh, tri_ids: np.ndarray) -> np.ndarray:
         val = A / 6.0 * ((g[i] ** 2).sum(axis=0) + (g[j] ** 2).sum(axis=0) + (g[i] * g[j]).sum(axis=0))
         np.add.at(out, V + eid, val)
     return out
+
+
+def vector_volume_diagonals(mesh, eps_r, inv_mu_r=None):
+    """Diagonals of the vector-nodal auxiliary space (columns d V + a, the
+    order of `vector_interpolation`): (curl, mass) with
+    curl = int (grad lam_a x e_d) . mu^-1 (grad lam_a x e_d) and
+    mass = int lam_a^2 e_d . eps e_d = V/10 eps_dd summed over the tets of
+    vertex a (int lam_a^2 = V/10).  eps_r and inv_mu_r are scalars (T,) or
+    3x3 tensors (3, 3, T); inv_mu_r defaults to 1."""
+    V, T = mesh.n_nodes, mesh.n_tets
+    g, vol = _tet_gradients(mesh.nodes, mesh.tets)          # (T, 4, 3)
+    eps = np.asarray(eps_r)
+    imu = np.ones(T) if inv_mu_r is None else np.asarray(inv_mu_r)
+    curl = np.zeros(3 * V, dtype=np.complex128)
+    mass = np.zeros(3 * V, dtype=np.complex128)
+    eye = np.eye(3)
+    for d in range(3):
+        ed = eye[d]
+        epsd = eps[d, d] if eps.ndim == 3 else eps                      # (T,)
+        for i in range(4):
+            c = np.cross(g[:, i], ed)                                  # (T, 3): curl(lam_i e_d)
+            if imu.ndim == 3:
+                cc = np.einsum("ta,abt,tb->t", c, imu, c)
+            else:
+                cc = (c * c).sum(axis=1) * imu
+            np.add.at(curl, d * V + mesh.tets[i], cc * vol)
+            np.add.at(mass, d * V + mesh.tets[i], epsd * vol / 10.0)
+    return curl, mass
+
+
+def vector_surface_diagonal(mesh, tri_ids: np.ndarray) -> np.ndarray:
+    """int_S |(lam_a e_d)_t|^2 dS over the given triangles (3 V,), columns
+    d V + a: A/6 (1 - n_d^2) per triangle of vertex a (int lam_a^2 = A/6,
+    n the unit normal)."""
+    V = mesh.n_nodes
+    out = np.zeros(3 * V)
+    if len(tri_ids) == 0:
+        return out
+    t = mesh.tris[:, tri_ids]
+    X = mesh.nodes[:, t]
+    n = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], axis=0)
+    area2 = np.sqrt((n ** 2).sum(axis=0))
+    nu = n / area2
+    A = area2 / 2
+    for d in range(3):
+        w = A / 6.0 * (1.0 - nu[d] ** 2)
+        for i in range(3):
+            np.add.at(out, d * V + t[i], w)
+    return out

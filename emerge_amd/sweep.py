@@ -19,6 +19,7 @@ import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
+import scipy.sparse as sp
 
 from . import bc as bcs
 from . import gradient
@@ -177,7 +178,8 @@ class FrequencySweep:
             self._assemble_materials(one, one, None)
         else:
             self._assemble_materials(eps_r, inv_mu_r, mu_r)
-        self._aux = None          # host arrays of the auxiliary gradient space, built on first use
+        self._aux = None          # host arrays of the auxiliary space, built on first use
+        self.vector_aux = True    # vector-nodal functions in the auxiliary space (set_vector_aux)
         self._dvol = None         # its volume diagonal for the assembled materials
         self._aux_on = False
         self._escalated = 0       # systems finished with complex128 inner solves in the last solve
@@ -199,6 +201,7 @@ class FrequencySweep:
         eye = np.eye(3)[:, :, None]
         if mu_r is None and e.ndim == 1:
             self.dm.assemble(e, np.asarray(inv_mu_r, dtype=np.complex128))
+            self._imu = np.asarray(inv_mu_r, dtype=np.complex128)
         else:
             et = e if e.ndim == 3 else e[None, None, :] * eye
             if mu_r is None:
@@ -211,6 +214,7 @@ class FrequencySweep:
                 mt = mt if mt.ndim == 3 else mt[None, None, :] * eye
             self.ctx.material_set_tensor(et, mt)
             self.ctx.assemble()
+            self._imu = np.moveaxis(np.linalg.inv(np.moveaxis(mt, 2, 0)), 0, 2)
         self.eps_r = e
         self._dvol = None
 
@@ -457,38 +461,76 @@ class FrequencySweep:
             self.port_norms = norms
 
     def aux_space(self):
-        """Auxiliary gradient space of the COCG preconditioner
-        ``D^-1 + G D_g^-1 G^T``: G = the discrete gradient of the second-order
-        nodal functions (``emerge_amd.gradient``) restricted to the reduced
-        rows and to the nodal functions that vanish on PEC (their gradients
-        have no tangential part there), columns in Morton order; D_g from the
-        exact volume integral int eps |grad phi|^2 and, per Robin term,
-        int_S |grad_t phi|^2 (second-order ABC corrections contribute none).
-        Returns (G csr, dvol, dsurf)."""
+        """Auxiliary space of the COCG preconditioner
+        ``D^-1 + A W D_a^-1 A^T`` (Hiptmair-Xu type, with diagonal solves):
+        A = [G | P] restricted to the reduced rows, with G the discrete
+        gradient of the second-order nodal functions and P the vector-nodal
+        interpolation of lam_a e_d (``emerge_amd.gradient``), both restricted
+        to the functions that vanish on PEC (vertices and edges of PEC
+        triangles dropped), columns in Morton order.  D_a per column:
+        dconst - k0^2 dvol + sum_s gamma_s dsurf[s] from exact integrals (G:
+        dconst 0, int eps |grad phi|^2, int_S |grad_t phi|^2; P: the curl-curl
+        int (grad lam_a x e_d) . mu^-1 (grad lam_a x e_d), int lam_a^2 eps_dd,
+        int_S lam_a^2 |e_d,t|^2; second-order ABC corrections contribute
+        none); W = 2 on the vertex hats, 1 elsewhere.  With the vector-nodal
+        part off (``self.vector_aux = False``) A = G, W = 1 (the plain
+        gradient-space correction).  Returns (A csr, dvol, dsurf, dconst, W)."""
         m = self.mesh
-        if self._aux is None:
+        V, nE = m.n_nodes, m.n_edges
+        vec = bool(self.vector_aux)
+        if self._aux is None or self._aux[5] != vec:
             f = self.basis
             G = gradient.discrete_gradient(m, f)
-            V = m.n_nodes
             drop = np.zeros(G.shape[1], dtype=bool)
+            pec_v = np.zeros(V, dtype=bool)
             for p in self.pec:
-                drop[np.unique(m.tris[:, p.tri_ids])] = True
+                pv = np.unique(m.tris[:, p.tri_ids])
+                drop[pv] = True
+                pec_v[pv] = True
                 drop[V + np.unique(m.tri_to_edge[:, p.tri_ids])] = True
-            cols = np.nonzero(~drop)[0]
             mid = 0.5 * (m.nodes[:, m.edges[0]] + m.nodes[:, m.edges[1]])
-            pos = np.concatenate([m.nodes, mid], axis=1)[:, cols]
-            cols = cols[_morton_order(pos, cols, float(np.median(m.edge_lengths)))]
-            Gr = G[self.solve_ids][:, cols].tocsr()
-            Gr.sort_indices()
+            pos = np.concatenate([m.nodes, mid], axis=1)
+            A = G
+            if vec:
+                P = gradient.vector_interpolation(m, f)
+                A = sp.hstack([G, P], format="csr")
+                drop = np.concatenate([drop, np.tile(pec_v, 3)])
+                pos = np.concatenate([pos, m.nodes, m.nodes, m.nodes], axis=1)
+            cols = np.nonzero(~drop)[0]
+            cols = cols[_morton_order(pos[:, cols], cols, float(np.median(m.edge_lengths)))]
+            Ar = A[self.solve_ids][:, cols].tocsr()
+            Ar.sort_indices()
+            nodal = cols < V + nE
             dsurf = np.zeros((len(self.surface), cols.shape[0]))
             for i, (kind, b) in enumerate(self.surface):
                 if kind == "robin":
-                    dsurf[i] = gradient.nodal_surface_diagonal(m, b.tri_ids)[cols]
-            self._aux = (Gr, cols, dsurf)
-        Gr, cols, dsurf = self._aux
+                    ds = gradient.nodal_surface_diagonal(m, b.tri_ids)
+                    if vec:
+                        ds = np.concatenate([ds, gradient.vector_surface_diagonal(m, b.tri_ids)])
+                    dsurf[i] = ds[cols]
+            wt = np.where(cols < V, 2.0, 1.0) if vec else np.ones(cols.shape[0])
+            self._aux = (Ar, cols, dsurf, nodal, wt, vec)
+        Ar, cols, dsurf, nodal, wt, vec = self._aux
         if self._dvol is None:
-            self._dvol = gradient.nodal_volume_diagonal(m, self.eps_r)[cols]
-        return Gr, self._dvol, dsurf
+            dv = gradient.nodal_volume_diagonal(m, self.eps_r)
+            dc = np.zeros(dv.shape[0], dtype=np.complex128)
+            if vec:
+                curl, mass = gradient.vector_volume_diagonals(m, self.eps_r, self._imu)
+                dv = np.concatenate([dv, mass])
+                dc = np.concatenate([dc, curl])
+            self._dvol = (dv[cols], dc[cols])
+        dvol, dconst = self._dvol
+        return Ar, dvol, dsurf, dconst, wt
+
+    def set_vector_aux(self, on: bool) -> None:
+        """Keep (True, the default) or leave out the vector-nodal functions of
+        the auxiliary space; the device copy is rebuilt on the next solve."""
+        if bool(on) != self.vector_aux:
+            self.vector_aux = bool(on)
+            self._dvol = None
+            if self._aux_on:
+                self._aux_on = False
+                self._set_aux(True)
 
     def _set_aux(self, on: bool) -> None:
         if on == self._aux_on:
@@ -497,7 +539,7 @@ class FrequencySweep:
             self.ctx.call("emg_aux_set", 0, None, None, None, None, None, None, None, 0, None)
             self._aux_on = False
             return
-        Gr, dvol, dsurf = self.aux_space()
+        Gr, dvol, dsurf, dconst, wt = self.aux_space()
         Gt = Gr.T.tocsr()
         Gt.sort_indices()
         gp, gi = np.ascontiguousarray(Gr.indptr, np.int32), np.ascontiguousarray(Gr.indices, np.int32)
@@ -508,6 +550,9 @@ class FrequencySweep:
         ds = np.ascontiguousarray(dsurf, np.float64)
         self.ctx.call("emg_aux_set", Gr.shape[1], ptr(gp), ptr(gi), ptr(gv), ptr(tp), ptr(ti), ptr(tv), ptr(dv),
                       ds.shape[0], ptr(ds) if ds.size else None)
+        dc = np.ascontiguousarray(dconst, np.complex128)
+        w = np.ascontiguousarray(wt, np.float64)
+        self.ctx.call("emg_aux_set_terms", ptr(dc), ptr(w))
         self._aux_on = True
 
     def voltage_line_probe(self, port, npts: int = 21):

@@ -286,6 +286,16 @@ int emg_pec_keep(emg_ctx* ctx, int64_t n_pec, const int32_t* pec_tris, int64_t* 
 int emg_aux_set(emg_ctx* ctx, int64_t n_nodal, const int32_t* g_ptr, const int32_t* g_idx, const double* g_val,
                 const int32_t* gt_ptr, const int32_t* gt_idx, const double* gt_val, const double* dvol,
                 int n_surf, const double* dsurf);
+/* Per-function terms of that auxiliary space (after emg_aux_set): the
+ * diagonal becomes D_g = dconst - k0^2 dvol + sum_s gamma_s dsurf[s] and the
+ * correction G W D_g^-1 G^T with W = diag(weight) (dconst complex n_nodal,
+ * weight real n_nodal; either may be NULL: 0 and 1).  Lets the space hold, next to the nodal
+ * gradients, the vector-nodal functions lam_a e_d interpolated into the
+ * Nedelec-2 space, whose diagonal has the frequency-independent curl-curl
+ * part dconst (auxiliary-space preconditioning of H(curl), Hiptmair-Xu).
+ * No counterpart in the reference (E/solver.py:437-527 is its iterative
+ * path with ILU); emg_aux_set clears the terms. */
+int emg_aux_set_terms(emg_ctx* ctx, const double* dconst, const double* weight);
 
 /* Dense element matrices of every tet of the context's mesh (isotropic
  * materials of emg_material_set): the 20x20 curl-curl K and mass M of

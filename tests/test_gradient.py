@@ -188,3 +188,59 @@ def test_vector_interpolation_constants_are_coordinate_gradients(mesh):
         x[:V] = m.nodes[d]
         a, b = P @ ones, G @ x
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("kind", ["scalar", "tensor"])
+def test_vector_diagonals_are_PtMP(mesh, kind):
+    """The vector-nodal diagonals equal diag(P^T K P) with K the oracle's
+    curl-curl matrix (exact integrals) and diag(P^T M P) with the exactly
+    integrated mass matrix, for scalar and general complex tensors."""
+    m, f = mesh
+    T = m.n_tets
+    rng = np.random.default_rng(4)
+    if kind == "scalar":
+        eps = rng.uniform(1, 4, T) * (1 - 0.01j)
+        imu = rng.uniform(0.5, 2, T) + 0j
+        eps_t = lambda t: eps[t] * np.eye(3)  # noqa: E731
+        imu_t = lambda t: np.eye(3) * imu[t]  # noqa: E731
+    else:
+        eps = (np.eye(3)[:, :, None] * rng.uniform(1, 4, (3, 1, T))
+               + 0.3 * (rng.standard_normal((3, 3, T)) + 1j * rng.standard_normal((3, 3, T))))
+        mu = (np.eye(3)[:, :, None] * rng.uniform(1, 3, (3, 1, T))
+              + 0.2 * (rng.standard_normal((3, 3, T)) + 1j * rng.standard_normal((3, 3, T))))
+        imu = np.moveaxis(np.linalg.inv(np.moveaxis(mu, 2, 0)), 0, 2)
+        eps_t = lambda t: eps[:, :, t]  # noqa: E731
+        imu_t = lambda t: imu[:, :, t]  # noqa: E731
+    rows, cols, kv, mv = [], [], [], []
+    for t in range(T):
+        vid = m.tets[:, t]
+        Kt, Mt = qd.element_matrices_quadrature(m.nodes[:, vid], vid, m.edges[:, m.tet_to_edge[:, t]],
+                                                m.tris[:, m.tet_to_tri[:, t]], imu_t(t), eps_t(t))
+        d = f.tet_to_field[:, t]
+        rows.append(np.repeat(d, 20)); cols.append(np.tile(d, 20)); kv.append(Kt.ravel()); mv.append(Mt.ravel())
+    shape = (f.n_field, f.n_field)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    K = sp.csr_matrix((np.concatenate(kv), (r, c)), shape=shape)
+    M = sp.csr_matrix((np.concatenate(mv), (r, c)), shape=shape)
+    P = gr.vector_interpolation(m, f)
+    curl, mass = gr.vector_volume_diagonals(m, eps, imu)
+    refk = np.asarray(P.multiply(K @ P).sum(axis=0)).ravel()
+    refm = np.asarray(P.multiply(M @ P).sum(axis=0)).ravel()
+    assert np.abs(curl - refk).max() <= 1e-9 * np.abs(refk).max()
+    assert np.abs(mass - refm).max() <= 1e-9 * np.abs(refm).max()
+
+
+def test_vector_surface_diagonal_is_PtSP(mesh):
+    m, f = mesh
+    tri_ids = m.boundary_tris()[::3]
+    rows, cols, vals = [], [], []
+    for it in tri_ids:
+        S = robin.ned2_tri_stiff(m.nodes[:, m.tris[:, it]], 1.0)
+        d = f.tri_to_field[:, it]
+        rows.append(np.repeat(d, 8)); cols.append(np.tile(d, 8)); vals.append(np.real(S).ravel())
+    S = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(f.n_field, f.n_field))
+    P = gr.vector_interpolation(m, f)
+    ref = np.asarray(P.multiply(S @ P).sum(axis=0)).ravel()
+    got = gr.vector_surface_diagonal(m, tri_ids)
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
