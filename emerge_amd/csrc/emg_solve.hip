@@ -35,6 +35,9 @@ struct Op {
     const int32_t* idx = nullptr;
     const cplx<R>* E = nullptr;    // matrix values (or the stiffness part)
     const cplx<R>* B = nullptr;    // optional mass part, scaled by -k2[f]
+    // complex128 COCG only: the mass part read as complex64 by the Krylov
+    // SpMV (half its bytes; the refinement's residual keeps B in FP64)
+    const float2* B32 = nullptr;
     const int32_t* sptr = nullptr; // optional surface entries, CSR over rows
     const int32_t* scol = nullptr;
     const int32_t* sbc = nullptr;
@@ -1021,7 +1024,13 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
     if constexpr (sizeof(A) > sizeof(R)) er = op.Er64[j];
     else if (sizeof(R) == sizeof(double) && op.Er64) e = cmk<R>(R(op.Er64[j]), R(0));
     else e = op.E[j];
-    const cplx<R> b = op.B ? op.B[j] : cmk<R>(R(0), R(0));
+    cplx<R> b = cmk<R>(R(0), R(0));
+    if (sizeof(R) == sizeof(double) && op.B32) {
+        const float2 t = op.B32[j];
+        b = cmk<R>(R(t.x), R(t.y));
+    } else if (op.B) {
+        b = op.B[j];
+    }
     const cplx<R>* zc = z + c * F + f0;
     cplx<R> zv[V];
     if constexpr (V % 2 == 0) {
@@ -2004,7 +2013,9 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: bad arguments");
     if (restart < 1 || (c->krylov_method == 0 && restart > 1000) || restart > 1000000 || max_cycles < 1 || !(tol > 0))
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: bad solver parameters");
-    if (precision != 0 && precision != 1) return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision must be 0 or 1");
+    if (precision < 0 || precision > 2) return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision must be 0, 1 or 2");
+    if (precision == 2 && c->krylov_method != 2)
+        return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision 2 (complex64 mass values) needs COCG");
     fblock = std::max(1, std::min(fblock, 32));
     hipStream_t s = c->stream;
     const int64_t n = c->n_red;
@@ -2094,11 +2105,18 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         op32.gval_d = c->aux_gval64.p;
         op32.tval_d = c->aux_tval64.p;
     }
+    // precision 2: complex128 Krylov vectors, the SpMV's symmetric mass read
+    // as complex64 (the FP64 refinement corrects the perturbation)
+    Op<double> op64m = op64s;
+    if (precision == 2) {
+        if (!c->have_bsym) return fail(c, EMG_ERR_STATE, "emg_sweep_solve: symmetric mass part not built");
+        op64m.B32 = reinterpret_cast<const float2*>(c->Bs32.p);
+    }
     SolveParams P;
     P.tol = tol;
     P.restart = restart;
     P.max_cycles = max_cycles;
-    P.inner_rel = (precision == 1 || c->krylov_method == 2) ? c->inner_rel32 : 1e-13;
+    P.inner_rel = (precision != 0 || c->krylov_method == 2) ? c->inner_rel32 : 1e-13;
     P.method = c->krylov_method;
     P.idr_s = c->idr_s;
     DBuf<double2> b64, x64, xt, coef;
@@ -2120,7 +2138,8 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
         BatchStats st;
         const int rc = precision == 1
                            ? dispatch_batch<float>(c, F, op32, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st)
-                           : dispatch_batch<double>(c, F, op64s, op64, k2 + done, gam.data(), nbc, b64.p, x64.p, P, st);
+                           : dispatch_batch<double>(c, F, precision == 2 ? op64m : op64s, op64, k2 + done, gam.data(),
+                                                    nbc, b64.p, x64.p, P, st);
         if (rc != EMG_OK) return rc;
         if (precision == 1 && c->krylov_method == 2 && c->escalate) {
             // systems the complex64 inner solves could not finish continue with

@@ -98,16 +98,20 @@ class SolverHIP:
         out._aux = self._aux
         return out
 
-    def attach(self, mesh, basis, solve_ids=None, pec_tris=None) -> "SolverHIP":
+    def attach(self, mesh, basis, solve_ids=None, pec_tris=None, vector: bool = True) -> "SolverHIP":
         """Give the solver the mesh and DOF maps of the systems it will see,
-        so that COCG adds the auxiliary gradient-space correction
-        ``G D_g^-1 G^T`` (D_g = diag(G^T A G) formed on the device per
-        matrix).  SolveRoutine hands A[ix_(solve_ids, solve_ids)]
-        (E/solver.py:1266-1268) with solve_ids ascending, the DOFs left by
-        the PEC elimination (ASM/assembler.py:274-307, 406-407): pass them,
-        or the PEC triangles that define them.  Nodal functions whose
-        gradient touches an eliminated DOF are left out."""
-        from .gradient import discrete_gradient
+        so that COCG adds the auxiliary-space correction ``A D_a^-1 A^T``
+        with A = [G | P] (discrete gradient of the second-order nodal
+        functions, and with ``vector`` the vector-nodal functions lam_a e_d
+        interpolated into the Nedelec-2 space) and D_a = diag(A^T A_s A)
+        formed on the device per matrix.  SolveRoutine hands
+        A[ix_(solve_ids, solve_ids)] (E/solver.py:1266-1268) with solve_ids
+        ascending, the DOFs left by the PEC elimination
+        (ASM/assembler.py:274-307, 406-407): pass them, or the PEC triangles
+        that define them.  Auxiliary functions that touch an eliminated DOF
+        are left out."""
+        import scipy.sparse as sp
+        from .gradient import discrete_gradient, vector_interpolation
         N = basis.n_field
         if solve_ids is None:
             t = np.asarray(pec_tris if pec_tris is not None else [], dtype=np.int64)
@@ -120,6 +124,8 @@ class SolverHIP:
         if np.any(np.diff(solve_ids) <= 0):
             raise ValueError("solve_ids must be ascending (the order SolveRoutine uses)")
         G = discrete_gradient(mesh, basis).tocsr()
+        if vector:
+            G = sp.hstack([G, vector_interpolation(mesh, basis)], format="csr")
         dropped = np.ones(N, dtype=bool)
         dropped[solve_ids] = False
         touches = np.asarray(abs(G[dropped]).sum(axis=0)).ravel() > 0

@@ -691,11 +691,12 @@ class FrequencySweep:
     def choose_precision(self, freqs) -> str:
         """Inner-solve precision for precision='auto': 'fp64' where
         ``needs_safe32`` finds the mesh fine against the wavelength (there the
-        complex64 inner solves need the FP64 stiffness and accumulation, carry
-        no bandwidth advantage and can stall: measured 2.70 s mixed against
-        2.07 s FP64 on the 221k-tet cavity, 507 s against 168 s on the
-        one-million-tet patch antenna), 'mixed' otherwise (10.5x FP64 on the
-        WR-90 sweep)."""
+        complex64 inner solves need the FP64 stiffness and accumulation and
+        about 20-30% more iterations, which eat their bandwidth advantage:
+        measured 1.20 s mixed, 1.21 s FP64 and 1.13 s 'm32' on the 221k-tet
+        cavity, 76.8 s, 61.2 s and 64.7 s for 8 points of the one-million-tet
+        patch antenna), 'mixed' otherwise (1.23x the FP64 batched sweep and
+        9.5x the plain FP64 path on the 201-point WR-90 sweep)."""
         return "fp64" if self.needs_safe32(freqs) else "mixed"
 
     def autotune_fblock(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
@@ -717,7 +718,7 @@ class FrequencySweep:
         nb = k2.shape[0]
         self._set_aux(method == "cocg" and precond == "aux")
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s), float(inner_rel32))
-        prec = {"fp64": 0, "mixed": 1}[precision]
+        prec = {"fp64": 0, "mixed": 1, "m32": 2}[precision]
         best, best_f = np.inf, 1
         timings = {}
         self.ctx.call("emg_solver_set_escalation", 0)
@@ -760,16 +761,20 @@ class FrequencySweep:
         (mixed COCG): keep the real stiffness in FP64 and accumulate the
         SpMV and the gradient-space correction in FP64; 'auto' turns it on
         when ``needs_safe32`` says the mesh is fine against the wavelength.
-        precision: 'fp64', 'mixed' (complex64 inner solves) or 'auto'
-        (``choose_precision``).  groups / blocks: excite the ports of each
+        precision: 'fp64', 'mixed' (complex64 inner solves), 'm32'
+        (complex128 inner solves whose SpMV reads the symmetric mass as
+        complex64: 20 instead of 28 bytes per entry, the FP64 refinement
+        removes the perturbation) or 'auto' (``choose_precision``).  groups / blocks: excite the ports of each
         group together (one system per frequency and group) and extract S per
         block of independent ports (``s_matrix``); fields are then keyed by
         (frequency index, group index)."""
         freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
         if precision == "auto":
             precision = self.choose_precision(freqs)
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64', 'mixed' or 'auto'")
+        if precision not in ("fp64", "mixed", "m32"):
+            raise ValueError("precision must be 'fp64', 'mixed', 'm32' or 'auto'")
+        if precision == "m32" and method != "cocg":
+            raise ValueError("precision 'm32' (complex64 mass values) is a COCG mode")
         if method not in ("gmres", "idr", "cocg"):
             raise ValueError("method must be 'gmres', 'idr' or 'cocg'")
         if precond not in ("aux", "jacobi"):
@@ -793,7 +798,7 @@ class FrequencySweep:
         k2, gam, coef = self.batch_coefficients(freqs, groups)
         self._escalated = 0
         run = dict(fblock=fblock, tol=tol, restart=restart, max_cycles=max_cycles, keep_fields=keep_fields)
-        prec = {"fp64": 0, "mixed": 1}[precision]
+        prec = {"fp64": 0, "mixed": 1, "m32": 2}[precision]
         aux = method == "cocg" and precond == "aux"
         if not self.frequency_dependent:
             self._set_aux(aux)
@@ -821,11 +826,11 @@ class FrequencySweep:
                            S=S, fp64_fallback=fell_back.reshape(-1, npt), fp64_escalated=self._escalated)
 
     def _solve_rows(self, k2, gam, coef, prec, fallback, run):
-        """`_run_rows`, then (mixed precision, fallback on) the rows whose
-        complex64 inner solves the refinement gave up solved again in FP64."""
+        """`_run_rows`, then (precision 'mixed' or 'm32', fallback on) the
+        rows the refinement gave up solved again in FP64."""
         x, relres, iters, ov = self._run_rows(k2, gam, coef, prec, **run)
         fell_back = np.zeros(k2.shape[0], dtype=bool)
-        if prec == 1 and fallback:
+        if prec != 0 and fallback:
             bad = np.nonzero(~(relres <= run["tol"]))[0]
             if bad.size:
                 xb, rb, ib, ob = self._run_rows(k2[bad], gam[bad] if gam.size else gam, coef[bad], 0, **run)

@@ -175,7 +175,9 @@ int emg_forcing_get(emg_ctx* ctx, double* out);
  * (MW/microwave_3d.py:773-794; E/solver.py:1241-1326).  precision 0 = FP64
  * inner solves, 1 = complex64 inner solves inside FP64 iterative refinement
  * (with COCG, systems whose complex64 inner solves stall are finished with
- * complex128 inner solves from their iterate: emg_sweep_escalated); the
+ * complex128 inner solves from their iterate: emg_sweep_escalated), 2 (COCG
+ * only) = complex128 inner solves whose SpMV reads the symmetric mass part
+ * as complex64 inside the same FP64 refinement; the
  * method is emg_solver_config's; fblock = max systems per batch (1..32,
  * matrix read once per batch).  x_out complex [nbatch][n_red]; relres_out =
  * final FP64 ||b - A x|| / ||b||. */

@@ -157,7 +157,7 @@ def test_sweep_idr_matches_superlu(sweep, precision, s):
     print(f"idr iters={res.iters.tolist()} relres={res.relres.max():.2e}")
 
 
-@pytest.mark.parametrize("precision,fblock", [("fp64", 1), ("fp64", 4), ("mixed", 4), ("mixed", 16)])
+@pytest.mark.parametrize("precision,fblock", [("fp64", 1), ("fp64", 4), ("mixed", 4), ("mixed", 16), ("m32", 4)])
 def test_sweep_cocg_matches_superlu(sweep, precision, fblock):
     print(f"cocg precision={precision} fblock={fblock}", flush=True)
     freqs = np.array([8e9, 10e9, 12e9]) if fblock > 1 else np.array([10e9])
@@ -185,7 +185,7 @@ def test_solver_plugin_cocg(precision):
     assert rep.aux["method"] == "cocg"
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "m32"])
 def test_cavity_lumped_port_matches_superlu(precision):
     """Lumped port (voltage-line S11) in a lossy PEC cavity: residual and S11
     against SuperLU with the oracle's S extraction."""
@@ -269,6 +269,26 @@ def test_sweep_cocg_aux_needs_fewer_iterations(sweep):
     assert np.all(ra.relres <= 1e-8) and np.all(rj.relres <= 1e-8)
     assert ra.iters.sum() * 3 < rj.iters.sum(), (ra.iters, rj.iters)
     assert np.abs(ra.S - rj.S).max() <= 1e-6
+
+
+def test_vector_nodal_space_needs_fewer_iterations(sweep):
+    """The vector-nodal functions in the auxiliary space (default) against
+    the gradient space alone: fewer COCG iterations on the waveguide, the same
+    S (both against SuperLU)."""
+    freqs = np.array([8.5e9, 11.5e9])
+    kw = dict(precision="fp64", tol=1e-8, fblock=2, restart=20000, max_cycles=30, method="cocg", inner_rel32=1e-2)
+    try:
+        sweep.set_vector_aux(False)
+        rg = sweep.solve(freqs, **kw)
+    finally:
+        sweep.set_vector_aux(True)
+    rv = sweep.solve(freqs, **kw)
+    assert np.all(rg.relres <= 1e-8) and np.all(rv.relres <= 1e-8)
+    print(f"iterations gradient space {rg.iters.ravel().tolist()}, with vector-nodal {rv.iters.ravel().tolist()}")
+    assert rv.iters.sum() * 1.5 < rg.iters.sum(), (rv.iters, rg.iters)
+    for i, f in enumerate(freqs):
+        o = wg_case.oracle_solution(float(f))
+        assert np.abs(rv.S[i] - o["S"]).max() <= 1e-6
 
 
 def test_mixed_escalates_to_fp64_inner_solves(sweep):
