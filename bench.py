@@ -103,15 +103,19 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
     rp = sw.solve(plain_f, precision="fp64", fblock=1, **kw)
     tp = time.perf_counter() - t0
     tuning = None
+    lanes = 0
+    fb_mixed = fblock
     if fblock == "auto":
-        fblock = sw.autotune_fblock(mixed_f, precision="mixed", method=kw_m["method"], precond=precond)
-        tuning = {str(k): round(v * 1e6, 3) for k, v in sw.tuning["seconds_per_system_iteration"].items()}
+        fblock, lanes = sw.autotune(mixed_f, precision="mixed", method=kw_m["method"], precond=precond)
+        tuning = {f"F{k[0]}_gr{k[1]}": round(v * 1e6, 3) for k, v in sw.tuning["seconds_per_system_iteration"].items()}
         print(f"[bench] sweep: autotuned fblock {fblock} (us per system-iteration {tuning})", file=sys.stderr,
               flush=True)
     print(f"[bench] sweep: fp64 plain {tp:.2f} s; mixed batched, {len(mixed_f)} points", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
-    rm = sw.solve(mixed_f, precision="mixed", fblock=fblock, **kw_m)
+    if tuning is not None:
+        fb_mixed = "auto"  # the tuned width and lane layout (cached)
+    rm = sw.solve(mixed_f, precision="mixed", fblock=fb_mixed, **kw_m)
     tm = time.perf_counter() - t0
     print(f"[bench] sweep: mixed {tm:.2f} s; fp64 batched at the same block width", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -144,6 +148,7 @@ def sweep_bench(n_plain: int, n_mixed: int, fblock: int, cpu_points: int, method
         "mixed_batched": {"points": len(mixed_f), "converged": ok_m, "seconds": round(tm, 3),
                           "freq_points_per_s": round(ok_m / tm, 4), "fblock": fblock,
                           "fblock_tuning_us_per_system_iteration": tuning,
+                          "spmv_entries_in_flight": lanes,
                           "fp64_escalated_systems": int(rm.fp64_escalated),
                           "mean_iters_per_system": float(rm.iters.mean()), "max_relres": float(rm.relres.max()),
                           "max_dS_vs_direct": dS.get("mixed_batched")},

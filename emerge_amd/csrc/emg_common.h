@@ -156,6 +156,7 @@ struct emg_ctx {
 
     // Krylov method of the solves (emg_solver_config)
     int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part
+    int spmv_gr = 0;              // COCG SpMV: entries of a row in flight (0 = default for the batch width)
     int idr_s = 4;
     double inner_rel32 = 1e-5;    // FP32 inner reduction per refinement step
 

@@ -1073,17 +1073,31 @@ __device__ __forceinline__ void cg_entry(const Op<R>& op, const cplx<R>* __restr
 // spread one system per lane, and those lanes load their p before the
 // gathers.  Control flow is uniform: every wavefront walks the same number
 // of row groups and lanes past the last row only add zeros.
-template <typename R, int F, typename A = R>
+// Lane layout of k_cg_spmv: V systems per lane (8 complex64 or 4 complex128
+// per lane, 64 bytes; 4 with FP64 accumulators), FG = F / V lanes per entry,
+// GR entries of a row in flight (GRX, or by default 8 at F >= 32 and 16
+// below), LPR = FG GR lanes per row and 64 / LPR rows per wavefront.
+template <typename R, int F, typename A, int GRX>
+struct SpmvShape {
+    static constexpr int VM = sizeof(A) == 4 ? 8 : 4;
+    static constexpr int V = F >= VM ? VM : F;
+    static constexpr int FG = F / V;
+    static constexpr int GW = 64 / FG;
+    static constexpr int GR = GRX > 0 ? GRX : (GW >= 16 ? (F >= 32 ? 8 : 16) : GW);
+    // a row's system owners need GR >= V lanes per entry group
+    static constexpr bool valid = GR >= V && GR <= GW && (GR & (GR - 1)) == 0;
+};
+
+template <typename R, int F, typename A = R, int GRX = 0>
 __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __restrict__ k2,
                                                         const cplx<R>* __restrict__ gam,
                                                         const cplx<R>* __restrict__ p, cplx<R>* __restrict__ q,
                                                         CgState st) {
-    // 8 complex64 or 4 complex128 per lane (64 bytes); 4 with FP64 accumulators
-    constexpr int VM = sizeof(A) == 4 ? 8 : 4;
-    constexpr int V = F >= VM ? VM : F;         // systems per lane
-    constexpr int FG = F / V;                   // lanes per entry
-    constexpr int GW = 64 / FG;                 // entry lanes per wavefront
-    constexpr int GR = GW >= 16 ? (F >= 32 ? 8 : 16) : GW;  // entries in flight per row (GR >= V)
+    using Sh = SpmvShape<R, F, A, GRX>;
+    static_assert(Sh::valid, "SpMV lane layout");
+    constexpr int V = Sh::V;                    // systems per lane
+    constexpr int FG = Sh::FG;                  // lanes per entry
+    constexpr int GR = Sh::GR;                  // entries in flight per row (GR >= V)
     constexpr int LPR = FG * GR;                // lanes per row
     constexpr int RPW = 64 / LPR;               // rows per wavefront
     constexpr int W = kCgThreads / 64;
@@ -1169,6 +1183,42 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
         }
         st.part[int64_t(blockIdx.x) * F + threadIdx.x] = sum;
     }
+}
+
+// k_cg_spmv with GR entries in flight per row (emg_solver_set_spmv_lanes;
+// 0 or a layout the batch width does not allow: the default)
+template <typename R, int F, typename A>
+void launch_cg_spmv(int gr, unsigned grid, hipStream_t s, const Op<R>& op, const R* k2, const cplx<R>* gam,
+                    const cplx<R>* p, cplx<R>* q, const CgState& cs) {
+    switch (gr) {
+        case 4:
+            if constexpr (SpmvShape<R, F, A, 4>::valid) {
+                k_cg_spmv<R, F, A, 4><<<grid, kCgThreads, 0, s>>>(op, k2, gam, p, q, cs);
+                return;
+            }
+            break;
+        case 8:
+            if constexpr (SpmvShape<R, F, A, 8>::valid) {
+                k_cg_spmv<R, F, A, 8><<<grid, kCgThreads, 0, s>>>(op, k2, gam, p, q, cs);
+                return;
+            }
+            break;
+        case 16:
+            if constexpr (SpmvShape<R, F, A, 16>::valid) {
+                k_cg_spmv<R, F, A, 16><<<grid, kCgThreads, 0, s>>>(op, k2, gam, p, q, cs);
+                return;
+            }
+            break;
+        case 32:
+            if constexpr (SpmvShape<R, F, A, 32>::valid) {
+                k_cg_spmv<R, F, A, 32><<<grid, kCgThreads, 0, s>>>(op, k2, gam, p, q, cs);
+                return;
+            }
+            break;
+        default:
+            break;
+    }
+    k_cg_spmv<R, F, A, 0><<<grid, kCgThreads, 0, s>>>(op, k2, gam, p, q, cs);
 }
 
 // alpha = rho / p^T q (one workgroup; partials of k_cg_spmv)
@@ -1722,9 +1772,9 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             for (int i = 0; i < blk; ++i) {
                 k_cg_pupd<R, F><<<gq, 256, 0, s>>>(nq, z.p, pv.p, cs);
                 if (safe)
-                    k_cg_spmv<R, F, double><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, pv.p, q.p, cs);
+                    launch_cg_spmv<R, F, double>(c->spmv_gr, ga, s, op, k2d.p, gam.p, pv.p, q.p, cs);
                 else
-                    k_cg_spmv<R, F><<<ga, kCgThreads, 0, s>>>(op, k2d.p, gam.p, pv.p, q.p, cs);
+                    launch_cg_spmv<R, F, R>(c->spmv_gr, ga, s, op, k2d.p, gam.p, pv.p, q.p, cs);
                 k_cg_alpha<F><<<1, kCgThreads, 0, s>>>(int(ga), cs);
                 k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0, int(aux));
                 precondition();
@@ -1855,6 +1905,15 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
     if (k < n) b[k] = float(a[k]);
 }
 }  // namespace
+
+extern "C" int emg_solver_set_spmv_lanes(emg_ctx* c, int entries_in_flight) {
+    EMG_CHECK_CTX(c);
+    const int g = entries_in_flight;
+    if (g != 0 && g != 4 && g != 8 && g != 16 && g != 32)
+        return fail(c, EMG_ERR_ARG, "emg_solver_set_spmv_lanes: %d not in {0, 4, 8, 16, 32}", g);
+    c->spmv_gr = g;
+    return EMG_OK;
+}
 
 extern "C" int emg_solver_set_safe32(emg_ctx* c, int on) {
     EMG_CHECK_CTX(c);

@@ -699,17 +699,33 @@ class FrequencySweep:
         9.5x the plain FP64 path on the 201-point WR-90 sweep)."""
         return "fp64" if self.needs_safe32(freqs) else "mixed"
 
-    def autotune_fblock(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
-                        candidates=(4, 8, 16, 32), iters: int = 200, idr_s: int = 4,
-                        inner_rel32: float = 1e-2) -> int:
-        """Frequency-block width with the least device time per
-        system-iteration on this mesh: each candidate F solves the first F
+    @staticmethod
+    def spmv_lane_options(F: int, precision: str, safe32: bool) -> list:
+        """Entries-in-flight values of the COCG SpMV that batch width F can
+        use (the device's layout rule: V = 8 complex64 or 4 complex128 (or
+        FP64-accumulated) systems per lane, F / V lanes per entry, GR in
+        {4, 8, 16, 32} with V <= GR <= 64 V / F); 0 stands for the
+        default."""
+        vm = 8 if precision == "mixed" and not safe32 else 4
+        v = min(F, vm)
+        gw = 64 // (F // v)
+        return [0] + [g for g in (4, 8, 16, 32) if v <= g <= gw]
+
+    def autotune(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
+                 candidates=(4, 8, 16, 32), iters: int = 200, idr_s: int = 4, inner_rel32: float = 1e-2,
+                 lanes: bool = True) -> tuple:
+        """Frequency-block width F and SpMV lane layout (entries of a row in
+        flight, hence rows per wavefront) with the least device time per
+        system-iteration on this mesh: each candidate solves the first F
         (frequency, port) systems of `freqs` for at most `iters` inner
         iterations (one refinement cycle, complex128 continuation off), timed
         after a warm-up run; time is divided by the iterations actually run.
-        The result is cached per (precision, method, precond)."""
+        Lane layouts are tried for COCG only (the other methods use other
+        kernels).  The result (F, entries in flight; 0 = default) is cached
+        per (precision, method, precond, safe mode)."""
         import time as _time
-        key = (precision, method, precond)
+        safe = bool(getattr(self, "_safe32", False))
+        key = (precision, method, precond, safe)
         cache = self.__dict__.setdefault("_fblock_tuned", {})
         if key in cache:
             return cache[key]
@@ -719,28 +735,40 @@ class FrequencySweep:
         self._set_aux(method == "cocg" and precond == "aux")
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s), float(inner_rel32))
         prec = {"fp64": 0, "mixed": 1, "m32": 2}[precision]
-        best, best_f = np.inf, 1
+        best, best_cfg = np.inf, (1, 0)
         timings = {}
         self.ctx.call("emg_solver_set_escalation", 0)
         try:
             for F in candidates:
+                opts = self.spmv_lane_options(F, precision, safe) if (lanes and method == "cocg") else [0]
                 rows = np.arange(F) % nb
                 run = dict(fblock=F, tol=1e-300, restart=iters if method != "gmres" else min(iters, 100),
                            max_cycles=1, keep_fields=False)
                 args = (k2[rows], gam[rows] if gam.size else gam, coef[rows], prec)
-                self._run_rows(*args, **run)
-                t0 = _time.perf_counter()
-                _, _, it, _ = self._run_rows(*args, **run)
-                dt = _time.perf_counter() - t0
-                per = dt / max(int(it.sum()), 1)
-                timings[F] = per
-                if per < best:
-                    best, best_f = per, F
+                for g in opts:
+                    self.ctx.call("emg_solver_set_spmv_lanes", int(g))
+                    self._run_rows(*args, **run)
+                    t0 = _time.perf_counter()
+                    _, _, it, _ = self._run_rows(*args, **run)
+                    dt = _time.perf_counter() - t0
+                    per = dt / max(int(it.sum()), 1)
+                    timings[(F, g)] = per
+                    if per < best:
+                        best, best_cfg = per, (F, g)
         finally:
             self.ctx.call("emg_solver_set_escalation", 1)
-        cache[key] = best_f
+            self.ctx.call("emg_solver_set_spmv_lanes", 0)
+        cache[key] = best_cfg
         self.tuning = dict(cache=dict(cache), seconds_per_system_iteration=timings)
-        return best_f
+        return best_cfg
+
+    def autotune_fblock(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
+                        candidates=(4, 8, 16, 32), iters: int = 200, idr_s: int = 4,
+                        inner_rel32: float = 1e-2) -> int:
+        """The frequency-block width of ``autotune`` (lane layout included in
+        the search)."""
+        return self.autotune(freqs, precision=precision, method=method, precond=precond, candidates=candidates,
+                             iters=iters, idr_s=idr_s, inner_rel32=inner_rel32)[0]
 
     def solve(self, freqs, precision: str = "mixed", tol: float = 1e-8, fblock=32, restart: int | None = None,
               max_cycles: int = 200, keep_fields: bool = True, method: str = "cocg", idr_s: int = 4,
@@ -786,13 +814,16 @@ class FrequencySweep:
         if safe32 == "auto":
             safe32 = self.needs_safe32(freqs)
         self.ctx.call("emg_solver_set_safe32", int(bool(safe32)))
+        self._safe32 = bool(safe32)
         npt = len(self.ports) if groups is None else len(groups)
         if self.frequency_dependent:
             # one frequency per device batch: each has its own E and B
             fblock = max(1, npt)
-        elif fblock == "auto":
-            fblock = self.autotune_fblock(freqs, precision=precision, method=method, precond=precond,
+        lanes = 0
+        if not self.frequency_dependent and fblock == "auto":
+            fblock, lanes = self.autotune(freqs, precision=precision, method=method, precond=precond,
                                           idr_s=idr_s, inner_rel32=inner_rel32)
+        self.ctx.call("emg_solver_set_spmv_lanes", int(lanes))
         self.ctx.call("emg_solver_config", {"gmres": 0, "idr": 1, "cocg": 2}[method], int(idr_s),
                       float(inner_rel32))
         k2, gam, coef = self.batch_coefficients(freqs, groups)

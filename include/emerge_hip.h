@@ -244,6 +244,13 @@ int emg_solver_set_escalation(emg_ctx* ctx, int on);
  * on near-gradient z and FP32 stiffness values would swamp the result.  No
  * reference counterpart (the reference solves in FP64). */
 int emg_solver_set_safe32(emg_ctx* ctx, int on);
+/* Lane layout of the COCG SpMV for the following solves: entries of a row in
+ * flight (4, 8, 16 or 32; 0 = the default for the batch width: 8 at 32
+ * systems, 16 below).  Fewer entries in flight put more rows in a
+ * wavefront.  A value the batch width cannot use (fewer entries than
+ * systems per lane, more than lanes per wavefront) runs the default.  Used
+ * by the host's autotuner; no reference counterpart. */
+int emg_solver_set_spmv_lanes(emg_ctx* ctx, int entries_in_flight);
 
 /* Mesh topology on the device (replaces Mesh3D._pre_update,
  * E/mesh3d.py:386-498): unique edges / triangles of the tets (sorted vertex

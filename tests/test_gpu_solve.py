@@ -339,9 +339,14 @@ def test_autotuned_block_width(sweep):
     """fblock='auto' times each candidate width on this mesh, picks one of
     them, and the tuned sweep still converges to the oracle's S."""
     freqs = np.linspace(8e9, 12e9, 12)
-    f = sweep.autotune_fblock(freqs, candidates=(4, 8, 16))
+    f, lanes = sweep.autotune(freqs, candidates=(4, 8, 16))
     assert f in (4, 8, 16)
-    assert set(sweep.tuning["seconds_per_system_iteration"]) == {4, 8, 16}
+    assert lanes in sweep.spmv_lane_options(f, "mixed", False)
+    tried = sweep.tuning["seconds_per_system_iteration"]
+    assert {F for F, _ in tried} == {4, 8, 16}
+    # every usable lane layout of every width was timed (F = 16 complex64:
+    # 8 systems per lane, 2 lanes per entry, 8 to 32 entries in flight)
+    assert {g for F, g in tried if F == 16} == set(sweep.spmv_lane_options(16, "mixed", False)) == {0, 8, 16, 32}
     res = sweep.solve(freqs[:3], precision="mixed", tol=1e-8, fblock="auto", restart=20000, max_cycles=30,
                       method="cocg", inner_rel32=1e-2)
     assert np.all(res.relres <= 1e-8)
