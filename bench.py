@@ -232,12 +232,19 @@ def config_bench(name: str, nf: int, fblock: int, precision: str = "fp64") -> di
                    dS11=float(abs(rm.S[0, 0, 0] - rf.S[0, 0, 0])),
                    field_rel_diff=float(np.linalg.norm(a - b) / np.linalg.norm(b)))
     elif name == "patch":
+        t_set = time.perf_counter()
         p = w.patch_antenna()
+        print(f"[bench] patch: mesh of {p.mesh.n_tets} tets built", file=sys.stderr, flush=True)
         sw = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, w.patch_bcs(p))
+        print(f"[bench] patch: matrices assembled ({time.perf_counter() - t_set:.1f} s of set-up)", file=sys.stderr,
+              flush=True)
         grid = np.linspace(2e9, 3e9, 51)
         freqs = grid[np.round(np.linspace(0, 50, nf)).astype(int)]
         kw["keep_fields"] = False
+        # a report per block of systems (a full sweep runs for minutes)
         t0 = time.perf_counter()
+        sw.progress = lambda done, total: print(f"[bench] patch: {done}/{total} systems solved "
+                                                f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
         r = sw.solve(freqs, precision=precision, fblock=fblock, **kw)
         dt = time.perf_counter() - t0
         ok = int(np.sum(np.all(r.relres <= 1e-8, axis=1)))

@@ -180,6 +180,7 @@ class FrequencySweep:
             self._assemble_materials(eps_r, inv_mu_r, mu_r)
         self._aux = None          # host arrays of the auxiliary space, built on first use
         self.vector_aux = True    # vector-nodal functions in the auxiliary space (set_vector_aux)
+        self.progress = None      # optional callable (systems done, systems) after each block
         self._dvol = None         # its volume diagonal for the assembled materials
         self._aux_on = False
         self._escalated = 0       # systems finished with complex128 inner solves in the last solve
@@ -875,6 +876,24 @@ class FrequencySweep:
         return x, relres, iters, ov, fell_back
 
     def _run_rows(self, k2, gam, coef, prec, fblock, tol, restart, max_cycles, keep_fields):
+        """emg_sweep_solve over the given (frequency, port) rows: one call, or
+        with ``self.progress`` (a callable taking (rows done, rows)) one call
+        per block of `fblock` rows with a report after each (the device
+        splits the rows into the same blocks either way)."""
+        nbatch = k2.shape[0]
+        cb = getattr(self, "progress", None)
+        if cb is not None and nbatch > int(fblock):
+            parts = []
+            for a in range(0, nbatch, int(fblock)):
+                b = min(nbatch, a + int(fblock))
+                parts.append(self._run_rows_once(k2[a:b], gam[a:b] if gam.size else gam, coef[a:b], prec,
+                                                 fblock, tol, restart, max_cycles, keep_fields))
+                cb(b, nbatch)
+            cat = lambda i: None if parts[0][i] is None else np.concatenate([q[i] for q in parts])  # noqa: E731
+            return cat(0), cat(1), cat(2), cat(3)
+        return self._run_rows_once(k2, gam, coef, prec, fblock, tol, restart, max_cycles, keep_fields)
+
+    def _run_rows_once(self, k2, gam, coef, prec, fblock, tol, restart, max_cycles, keep_fields):
         """One emg_sweep_solve over the given (frequency, port) rows."""
         nbatch = k2.shape[0]
         k2 = np.ascontiguousarray(k2)
