@@ -6,7 +6,8 @@
 // owns a chunk of consecutive CSR rows.  Every lane takes one incidence (tet,
 // canonical row u) of those rows, computes the tet's canonical geometry and
 // row u of the element matrices into LDS; then every CSR slot of the chunk is
-// summed over its precomputed contributor list (lane, column), which is in
+// summed over its precomputed contributor list (LDS offsets of (lane,
+// column) values), which is in
 // increasing tet order -- the order in which scipy's COO->CSR duplicate sum
 // adds the reference's COO entries.  Results are bitwise reproducible (no
 // atomics) and no element matrix or COO list ever goes to HBM.  HBM traffic
@@ -28,31 +29,32 @@ using namespace emg;
 // (LDS carry), which adds the carries in lane order.  Every lane does the
 // same work whatever the slot sizes.  The finished slot values are held in
 // registers until every lane has read the element rows, then written into
-// the same LDS (the chunk's slots are one contiguous CSR range of at most
-// kAsmMaxSlots entries) and copied out with coalesced 16-byte stores: the
-// wavefront writes 1 KB of consecutive values per store instead of 64
-// scattered 16-byte pieces.
+// the same LDS one component at a time (the chunk's slots are one
+// contiguous CSR range of at most kAsmMaxSlots entries) and copied out with
+// coalesced stores: the wavefront writes one run of consecutive values per
+// store instead of 64 scattered pieces.
 //
 // kRealK: every tet's 1/mu_r is real (the common case), so the curl-curl
 // values are real: that pass stages its rows already multiplied by the
 // lane's real factor and sums one 8-byte LDS read per code (the imaginary
 // parts are exact zeros).
 template <bool kRealK>
-__global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_assemble_rows(
+__global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRealK ? 3 : 2, kRealK ? 3 : 2))) void k_assemble_rows(
     const Chunk* __restrict__ chunks, int64_t n_chunks, const int32_t* __restrict__ inc_tet,
     const uint8_t* __restrict__ inc_u, const int4* __restrict__ inc_vid, const uint16_t* __restrict__ grp_slot,
     const uint16_t* __restrict__ contrib, const double* __restrict__ xyz, int64_t V,
     const double2* __restrict__ eps, const double2* __restrict__ inu, double2* __restrict__ Ev,
     double2* __restrict__ Bv) {
     // element rows of every lane (20 x kAsmThreads doubles, 40 KB) while the
-    // sums run; the chunk's finished slot values (kAsmMaxSlots double2,
-    // 64 KB) afterwards
-    __shared__ double2 obuf[kAsmMaxSlots];
+    // sums run; one component of the chunk's finished slot values
+    // (at most kAsmMaxSlots doubles) afterwards.  49 KB of LDS in all: three
+    // workgroups per CU.
+    __shared__ double sbuf[20 * kAsmThreads];
     __shared__ double2 fac[kAsmThreads];       // complex row factor of every lane
     __shared__ double2 carry[kAsmThreads];     // partial sum of a slot continued from the previous lane
     __shared__ int closed[kAsmThreads];        // whether that slot ends in this lane's range
-    static_assert(20 * kAsmThreads * sizeof(double) <= kAsmMaxSlots * sizeof(double2), "stage fits obuf");
-    double* stage = reinterpret_cast<double*>(obuf);  // stage[c * kAsmThreads + lane]
+    static_assert(kAsmMaxSlots <= 20 * kAsmThreads, "slot values fit the stage");
+    double* stage = sbuf;  // stage[c * kAsmThreads + lane]
 
     // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so
     // give each XCD a contiguous range of chunks (neighbouring rows share
@@ -177,10 +179,9 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
                     const int r = b0 + j;
-                    const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
-                    const int ln = (code >> 5) & 0xff;
-                    xs[j] = stage[(code & 31) * kAsmThreads + ln];
-                    if constexpr (!real) fs[j] = fac[ln];
+                    const uint32_t off = (cv[r >> 1] >> (16 * (r & 1))) & kCodeOffset;
+                    xs[j] = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(stage) + off);
+                    if constexpr (!real) fs[j] = fac[(off >> 3) & (kAsmThreads - 1)];
                 }
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
@@ -207,24 +208,7 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(2, 
                 closed[tid] = pre ? 0 : 1;  // 0: the whole range lies inside that slot
             }
         }
-        __syncthreads();  // element rows read, carries written: obuf may now take the slot values
-        // Every code writes its running sum to its slot (the slot's last code
-        // writes last, in program order, so the final value stays); codes of
-        // the continued prefix write to this lane's own fac entry, which is
-        // no longer read in this pass.
-        if (active) {
-            int s = slot_first;
-            bool p = cont;
-#pragma unroll
-            for (int r = 0; r < 20; ++r) {
-                const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
-                const bool last = (code & kCodeLast) != 0;
-                double2* dst = p ? &fac[tid] : &obuf[s];
-                *dst = res[r];
-                p = p && !last;
-                s += last ? 1 : 0;
-            }
-        }
+        __syncthreads();  // element rows read, carries written: sbuf may now take the slot values
         // a slot that starts in this lane's range and runs past it: add the
         // following lanes' partial sums in order (the chunk's last code
         // always ends a slot, so the walk stops inside the chunk)
@@ -235,28 +219,51 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(2, 
                 im += cq.y;
                 if (closed[q]) break;
             }
-            obuf[slot] = make_double2(re, im);
         }
-        __syncthreads();
-        // copy-out: all (at most kAsmMaxSlots / kAsmThreads = 16) LDS reads
-        // of a lane first, then its coalesced stores
+        // One component of the chunk's slot values at a time (8 bytes per
+        // slot, so they fit the stage's LDS): the last code of every slot
+        // that starts in this lane's range writes it (a masked store), then
+        // the chunk leaves LDS with coalesced stores -- 16-byte (v, 0) pairs
+        // for the real curl-curl pass, the real then the imaginary halves of
+        // the complex values otherwise.
         double2* out = (pass == 0 ? Ev : Bv) + ch.slot0;
-        {
-            constexpr int kPer = kAsmMaxSlots / kAsmThreads;
-            double vr[kPer], vi[kPer];
+        auto emit = [&](auto comp_tag) __attribute__((always_inline)) {
+            constexpr int comp = decltype(comp_tag)::value;
+            if (active) {
+                int s = slot_first;
+                bool p = cont;
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) {
-                const double2 t = obuf[j * kAsmThreads + tid];  // < kAsmMaxSlots: in bounds
-                vr[j] = t.x;
-                vi[j] = t.y;
+                for (int r = 0; r < 20; ++r) {
+                    const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                    const bool last = (code & kCodeLast) != 0;
+                    if (last && !p) sbuf[s] = comp == 0 ? res[r].x : res[r].y;
+                    p = p && !last;
+                    s += last ? 1 : 0;
+                }
             }
+            if (tail_open && !pre) sbuf[slot] = comp == 0 ? re : im;
+            __syncthreads();
+            // all (at most kAsmMaxSlots / kAsmThreads = 16) LDS reads of a
+            // lane first, then its stores
+            constexpr int kPer = kAsmMaxSlots / kAsmThreads;
+            double v[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) v[j] = sbuf[j * kAsmThreads + tid];  // < kAsmMaxSlots: in bounds
 #pragma unroll
             for (int j = 0; j < kPer; ++j) {
                 const int i = j * kAsmThreads + tid;
-                if (i < ch.nslot) out[i] = make_double2(vr[j], vi[j]);
+                if (i < ch.nslot) {
+                    if constexpr (real) {
+                        out[i] = make_double2(v[j], 0.0);
+                    } else {
+                        reinterpret_cast<double*>(out + i)[comp] = v[j];
+                    }
+                }
             }
-        }
-        __syncthreads();  // obuf is the next pass's stage
+            __syncthreads();  // sbuf is the next component's or pass's buffer
+        };
+        emit(std::integral_constant<int, 0>{});
+        if constexpr (!real) emit(std::integral_constant<int, 1>{});
     };
     run_pass(std::integral_constant<int, 0>{});
     run_pass(std::integral_constant<int, 1>{});
@@ -396,8 +403,9 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows_tensor(
 #pragma unroll
             for (int r = 0; r < 20; ++r) {
                 const uint32_t code = (cw[r >> 1] >> (16 * (r & 1))) & 0xffffu;
-                const int ln = (code >> 5) & 0xff;
-                acc = __builtin_fma(fac[ln], stage[code & 31][ln], acc);
+                const uint32_t off = code & kCodeOffset;
+                const double x = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(&stage[0][0]) + off);
+                acc = __builtin_fma(fac[(off >> 3) & (kAsmThreads - 1)], x, acc);
                 if (code & kCodeLast) {
                     if (first && cont) {
                         carry[tid] = acc;

@@ -53,8 +53,13 @@ struct Chunk {
 };
 
 constexpr int kAsmThreads = 256;
-constexpr uint16_t kCodeLast = 0x8000;  // contributor code flag: last of its CSR slot
-constexpr uint16_t kCodeCont = 0x2000;  // flag: group start that continues the previous group's slot
+// Contributor code (16 bits): bits 3-15 hold the LDS byte offset
+// 8 * (v * kAsmThreads + lane) of element-row value v of incidence lane in
+// the assembly kernels' stage (at most 8 * (20 * 256) - 8 = 40952), bits 0-1
+// the flags below.
+constexpr uint16_t kCodeLast = 0x0001;  // contributor code flag: last of its CSR slot
+constexpr uint16_t kCodeCont = 0x0002;  // flag: group start that continues the previous group's slot
+constexpr uint32_t kCodeOffset = 0xfff8u;  // mask of the stage byte offset
 constexpr int kAsmMaxSlots = 4096;
 
 }  // namespace emg

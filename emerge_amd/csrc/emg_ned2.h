@@ -295,6 +295,21 @@ __device__ __forceinline__ void ranks4(const int* w, int* rk) {
     for (int k = 0; k < 4; ++k) rk[k] = (w[0] < w[k]) + (w[1] < w[k]) + (w[2] < w[k]) + (w[3] < w[k]);
 }
 
+// x / 6 correctly rounded, as IEEE division gives it, in three operations
+// instead of a full division sequence.  q = RN(x r) with r = RN(1/6) is
+// within one unit in the last place of x/6; e = x - 6q is exact (one FMA);
+// q + e r then differs from x/6 by at most 2^-54 of a unit in the last place
+// before its single rounding.  x/6 = (x/2)/3 never lies closer than 1/6 of
+// a unit to a rounding midpoint (a midpoint times 3 is an odd multiple of
+// 3 * 2^-55 in the binade's units, which no double x/2 equals), so the final
+// rounding returns RN(x/6) for every x whose quotient is a normal number.
+__device__ __forceinline__ double div6(double x) {
+    constexpr double r = 0x1.5555555555555p-3;  // RN(1/6)
+    const double q = x * r;
+    const double e = __builtin_fma(-q, 6.0, x);
+    return __builtin_fma(e, r, q);
+}
+
 // Canonical gradient vectors gl (cofactors, 6V grad lam) and their pairwise
 // crosses c, for material tensors (the Gram products then carry the tensor).
 struct TetVecs {
@@ -329,10 +344,14 @@ __device__ __forceinline__ void tet_vectors(const double (&XL)[4][3], const int 
     gll[1][2] = x1*y3 - x1*y4 - x3*y1 + x3*y4 + x4*y1 - x4*y3;
     gll[2][2] = -x1*y2 + x1*y4 + x2*y1 - x2*y4 - x4*y1 + x4*y2;
     gll[3][2] = x1*y2 - x1*y3 - x2*y1 + x2*y3 + x3*y1 - x3*y2;
-    const double vol = fabs(-x1*y2*z3/6 + x1*y2*z4/6 + x1*y3*z2/6 - x1*y3*z4/6 - x1*y4*z2/6 + x1*y4*z3/6
-                            + x2*y1*z3/6 - x2*y1*z4/6 - x2*y3*z1/6 + x2*y3*z4/6 + x2*y4*z1/6 - x2*y4*z3/6
-                            - x3*y1*z2/6 + x3*y1*z4/6 + x3*y2*z1/6 - x3*y2*z4/6 - x3*y4*z1/6 + x3*y4*z2/6
-                            + x4*y1*z2/6 - x4*y1*z3/6 - x4*y2*z1/6 + x4*y2*z3/6 + x4*y3*z1/6 - x4*y3*z2/6);
+    // the reference's 24 terms (x*y*z)/6 in its order; div6 rounds each
+    // quotient exactly as the division does
+    const double vol = fabs(-div6(x1*y2*z3) + div6(x1*y2*z4) + div6(x1*y3*z2) - div6(x1*y3*z4)
+                            - div6(x1*y4*z2) + div6(x1*y4*z3) + div6(x2*y1*z3) - div6(x2*y1*z4)
+                            - div6(x2*y3*z1) + div6(x2*y3*z4) + div6(x2*y4*z1) - div6(x2*y4*z3)
+                            - div6(x3*y1*z2) + div6(x3*y1*z4) + div6(x3*y2*z1) - div6(x3*y2*z4)
+                            - div6(x3*y4*z1) + div6(x3*y4*z2) + div6(x4*y1*z2) - div6(x4*y1*z3)
+                            - div6(x4*y2*z1) + div6(x4*y2*z3) + div6(x4*y3*z1) - div6(x4*y3*z2));
     T.inv6v = 1.0 / (6.0 * vol);
     // canonical frame: gl[c] / X[c] belong to the local vertex of rank c
     double gl[4][3], X[4][3];
@@ -352,11 +371,12 @@ __device__ __forceinline__ void tet_vectors(const double (&XL)[4][3], const int 
 #pragma unroll
     for (int e = 0; e < 6; ++e) {
         const int a = kEdge[e][0], b = kEdge[e][1];
-        T.c[e][0] = gl[a][1] * gl[b][2] - gl[a][2] * gl[b][1];
-        T.c[e][1] = gl[a][2] * gl[b][0] - gl[a][0] * gl[b][2];
-        T.c[e][2] = gl[a][0] * gl[b][1] - gl[a][1] * gl[b][0];
+        // (no rounding order to follow here: fused products)
+        T.c[e][0] = __builtin_fma(gl[a][1], gl[b][2], -(gl[a][2] * gl[b][1]));
+        T.c[e][1] = __builtin_fma(gl[a][2], gl[b][0], -(gl[a][0] * gl[b][2]));
+        T.c[e][2] = __builtin_fma(gl[a][0], gl[b][1], -(gl[a][1] * gl[b][0]));
         const double dx = X[a][0] - X[b][0], dy = X[a][1] - X[b][1], dz = X[a][2] - X[b][2];
-        T.d[e] = sqrt(dx * dx + dy * dy + dz * dz);
+        T.d[e] = sqrt(__builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz)));
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -373,12 +393,14 @@ __device__ __forceinline__ void tet_geometry(const double (&XL)[4][3], const int
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = a; b < 4; ++b)
-            G.g[gram_index(a, b)] = T.gl[a][0] * T.gl[b][0] + T.gl[a][1] * T.gl[b][1] + T.gl[a][2] * T.gl[b][2];
+            G.g[gram_index(a, b)] =
+                __builtin_fma(T.gl[a][0], T.gl[b][0], __builtin_fma(T.gl[a][1], T.gl[b][1], T.gl[a][2] * T.gl[b][2]));
 #pragma unroll
     for (int e = 0; e < 6; ++e)
 #pragma unroll
         for (int f = e; f < 6; ++f)
-            G.h[cross_gram_index(e, f)] = T.c[e][0] * T.c[f][0] + T.c[e][1] * T.c[f][1] + T.c[e][2] * T.c[f][2];
+            G.h[cross_gram_index(e, f)] =
+                __builtin_fma(T.c[e][0], T.c[f][0], __builtin_fma(T.c[e][1], T.c[f][1], T.c[e][2] * T.c[f][2]));
 #pragma unroll
     for (int e = 0; e < 6; ++e) G.d[e] = T.d[e];
 }

@@ -97,7 +97,8 @@ __device__ __forceinline__ void canonical_map(const int* w, int* umap) {
 // One workgroup per chunk.  Per incidence: its tet, vertex ids and canonical
 // row u (incidences stably reordered by u inside the chunk, so that a
 // wavefront evaluates few distinct element rows).  The chunk's 20*ninc
-// contributor codes (incidence lane << 5 | canonical column) are grouped by
+// contributor codes (stage byte offset 8 * (canonical column * kAsmThreads +
+// incidence lane), flags in the low bits) are grouped by
 // CSR slot, each slot's list in rank order within the row (= increasing tet
 // id), which fixes the summation order.  Flags: kCodeLast on the last code of
 // every slot, kCodeCont on a code at a group start (index % 20 == 0) that
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_schedule(
     if (l < ch.ninc) {
         for (int v = 0; v < 20; ++v) {
             const int idx = cnt[off[v]] + atomicAdd(&fillc[off[v]], 1);
-            tcode[idx] = uint16_t((pos << 5) | v);
+            tcode[idx] = uint16_t((v * kAsmThreads + pos) << 3);
             trank[idx] = uint8_t(rank);
         }
     }
