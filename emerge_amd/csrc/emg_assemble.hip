@@ -222,12 +222,16 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRe
         }
         // One component of the chunk's slot values at a time (8 bytes per
         // slot, so they fit the stage's LDS): the last code of every slot
-        // that starts in this lane's range writes it (a masked store), then
-        // the chunk leaves LDS with coalesced stores -- 16-byte (v, 0) pairs
-        // for the real curl-curl pass, the real then the imaginary halves of
-        // the complex values otherwise.
+        // that starts in this lane's range writes it (a masked store), and
+        // every lane reads back its share of the chunk's values (at most
+        // kAsmMaxSlots / kAsmThreads = 16).  The real curl-curl pass then
+        // stores (v, 0) pairs; the complex pass keeps the real parts in
+        // registers, stages the imaginary parts the same way and stores
+        // whole 16-byte values: every store instruction writes one run of
+        // consecutive full values.
         double2* out = (pass == 0 ? Ev : Bv) + ch.slot0;
-        auto emit = [&](auto comp_tag) __attribute__((always_inline)) {
+        constexpr int kPer = kAsmMaxSlots / kAsmThreads;
+        auto stage_component = [&](auto comp_tag, double (&v)[kPer]) __attribute__((always_inline)) {
             constexpr int comp = decltype(comp_tag)::value;
             if (active) {
                 int s = slot_first;
@@ -243,27 +247,23 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRe
             }
             if (tail_open && !pre) sbuf[slot] = comp == 0 ? re : im;
             __syncthreads();
-            // all (at most kAsmMaxSlots / kAsmThreads = 16) LDS reads of a
-            // lane first, then its stores
-            constexpr int kPer = kAsmMaxSlots / kAsmThreads;
-            double v[kPer];
 #pragma unroll
             for (int j = 0; j < kPer; ++j) v[j] = sbuf[j * kAsmThreads + tid];  // < kAsmMaxSlots: in bounds
-#pragma unroll
-            for (int j = 0; j < kPer; ++j) {
-                const int i = j * kAsmThreads + tid;
-                if (i < ch.nslot) {
-                    if constexpr (real) {
-                        out[i] = make_double2(v[j], 0.0);
-                    } else {
-                        reinterpret_cast<double*>(out + i)[comp] = v[j];
-                    }
-                }
-            }
             __syncthreads();  // sbuf is the next component's or pass's buffer
         };
-        emit(std::integral_constant<int, 0>{});
-        if constexpr (!real) emit(std::integral_constant<int, 1>{});
+        double vr[kPer], vi[kPer];
+        stage_component(std::integral_constant<int, 0>{}, vr);
+        if constexpr (real) {
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) vi[j] = 0.0;
+        } else {
+            stage_component(std::integral_constant<int, 1>{}, vi);
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = j * kAsmThreads + tid;
+            if (i < ch.nslot) out[i] = make_double2(vr[j], vi[j]);
+        }
     };
     run_pass(std::integral_constant<int, 0>{});
     run_pass(std::integral_constant<int, 1>{});
