@@ -53,7 +53,8 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRe
     __shared__ double2 fac[kAsmThreads];       // complex row factor of every lane
     __shared__ double2 carry[kAsmThreads];     // partial sum of a slot continued from the previous lane
     __shared__ int closed[kAsmThreads];        // whether that slot ends in this lane's range
-    static_assert(kAsmMaxSlots <= 20 * kAsmThreads, "slot values fit the stage");
+    __shared__ int wslot[kAsmThreads / 64 + 1];
+    static_assert(kAsmThreads % 64 == 0, "whole wavefronts");
     double* stage = sbuf;  // stage[c * kAsmThreads + lane]
 
     // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so
@@ -112,6 +113,11 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRe
     }
     const bool cont = (cw[0] & kCodeCont) != 0;
     const bool tail_open = active && (cw[9] & (uint32_t(kCodeLast) << 16)) == 0;
+    // first slot that starts in each wavefront's code range (wslot[4] =
+    // nslot); read after the first barrier
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) wslot[wave] = active ? slot_first + (cont ? 1 : 0) : ch.nslot;
+    if (tid == 0) wslot[kAsmThreads / 64] = ch.nslot;
 
     // pass 0: curl-curl -> Ev; pass 1: mass -> Bv
     auto run_pass = [&](auto pass_tag) __attribute__((always_inline)) {
@@ -220,17 +226,22 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRe
                 if (closed[q]) break;
             }
         }
-        // One component of the chunk's slot values at a time (8 bytes per
-        // slot, so they fit the stage's LDS): the last code of every slot
-        // that starts in this lane's range writes it (a masked store), and
-        // every lane reads back its share of the chunk's values (at most
-        // kAsmMaxSlots / kAsmThreads = 16).  The real curl-curl pass then
-        // stores (v, 0) pairs; the complex pass keeps the real parts in
-        // registers, stages the imaginary parts the same way and stores
-        // whole 16-byte values: every store instruction writes one run of
-        // consecutive full values.
+        // Each wavefront writes out the slots that start in its own code
+        // range, [wslot[w], wslot[w + 1]): one component at a time (8 bytes
+        // per slot) it stages them in its own 20 x 64 stage entries (room
+        // for 1280 slots; a wavefront's range holds at most 1280 codes), the
+        // last code of every slot that starts in a lane's range writing it (a
+        // masked store), reads them back in slot order and stores whole
+        // 16-byte values, (v, 0) for the real curl-curl pass: every store
+        // instruction writes one run of consecutive values.  No other
+        // wavefront touches these entries until the next pass's element rows
+        // (written by this wavefront after its reads), so this phase needs
+        // no workgroup barrier.
         double2* out = (pass == 0 ? Ev : Bv) + ch.slot0;
-        constexpr int kPer = kAsmMaxSlots / kAsmThreads;
+        const int wbase = wslot[wave];
+        const int wn = wslot[wave + 1] - wbase;
+        constexpr int kPer = 20;  // 1280 / 64
+        auto at = [&](int j) __attribute__((always_inline)) { return (j >> 6) * kAsmThreads + (wave << 6) + (j & 63); };
         auto stage_component = [&](auto comp_tag, double (&v)[kPer]) __attribute__((always_inline)) {
             constexpr int comp = decltype(comp_tag)::value;
             if (active) {
@@ -240,29 +251,31 @@ __global__ __launch_bounds__(kAsmThreads) __attribute__((amdgpu_waves_per_eu(kRe
                 for (int r = 0; r < 20; ++r) {
                     const uint32_t code = (cv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
                     const bool last = (code & kCodeLast) != 0;
-                    if (last && !p) sbuf[s] = comp == 0 ? res[r].x : res[r].y;
+                    if (last && !p) sbuf[at(s - wbase)] = comp == 0 ? res[r].x : res[r].y;
                     p = p && !last;
                     s += last ? 1 : 0;
                 }
             }
-            if (tail_open && !pre) sbuf[slot] = comp == 0 ? re : im;
-            __syncthreads();
+            if (tail_open && !pre) sbuf[at(slot - wbase)] = comp == 0 ? re : im;
+            // the lanes of one wavefront read each other's writes: LDS
+            // operations of a wavefront complete in order
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) v[j] = sbuf[j * kAsmThreads + tid];  // < kAsmMaxSlots: in bounds
-            __syncthreads();  // sbuf is the next component's or pass's buffer
+            for (int t = 0; t < kPer; ++t) v[t] = (t * 64 + lane < wn) ? sbuf[t * kAsmThreads + (wave << 6) + lane] : 0.0;
+            __builtin_amdgcn_wave_barrier();
         };
         double vr[kPer], vi[kPer];
         stage_component(std::integral_constant<int, 0>{}, vr);
         if constexpr (real) {
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) vi[j] = 0.0;
+            for (int t = 0; t < kPer; ++t) vi[t] = 0.0;
         } else {
             stage_component(std::integral_constant<int, 1>{}, vi);
         }
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int i = j * kAsmThreads + tid;
-            if (i < ch.nslot) out[i] = make_double2(vr[j], vi[j]);
+        for (int t = 0; t < kPer; ++t) {
+            const int j = t * 64 + lane;
+            if (j < wn) out[wbase + j] = make_double2(vr[t], vi[t]);
         }
     };
     run_pass(std::integral_constant<int, 0>{});
