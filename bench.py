@@ -409,6 +409,38 @@ def farfield_bench(n: int = 60, reps: int = 5) -> dict:
     return res
 
 
+def hbm_probe(nbytes: int = 11_514_094_384, reps: int = 10) -> dict:
+    """Achievable HBM rates for plain streams as large as the 60^3 cube's
+    E and B values (11.51 GB): torch's fill (write only), copy (half read,
+    half write) and sum (read only) kernels, timed with events."""
+    import torch
+    dev = torch.device("cuda:0")
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    n = nbytes // 8
+    x = torch.empty(n, dtype=torch.float64, device=dev)
+    t_fill = timed(lambda: x.fill_(1.0))
+    t_sum = timed(lambda: torch.sum(x))
+    del x
+    y = torch.ones(n // 2, dtype=torch.float64, device=dev)
+    z = torch.empty_like(y)
+    t_copy = timed(lambda: z.copy_(y))
+    gb = n * 8 / 1e9
+    return {"bytes": n * 8, "fill_ms": round(t_fill, 4), "fill_TBps": round(gb / t_fill, 3),
+            "copy_ms": round(t_copy, 4), "copy_TBps": round(gb / t_copy, 3),
+            "sum_ms": round(t_sum, 4), "sum_TBps": round(gb / t_sum, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,6 +466,9 @@ def main() -> int:
                     help="COCG preconditioner: Jacobi + auxiliary gradient space, or Jacobi alone")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="counter calibration: only run 3 device-to-device copies of this many MiB")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only time plain device fill / copy / sum streams of the assembly's output size "
+                         "(the achievable HBM write and read rates)")
     args = ap.parse_args()
     args.fblock = args.fblock if args.fblock == "auto" else int(args.fblock)
     if args.restart is None:
@@ -454,6 +489,10 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": nb}), flush=True)
+        return 0
+
+    if args.hbm_probe:
+        print(json.dumps(hbm_probe()), flush=True)
         return 0
 
     from emerge_amd._lib import Context
