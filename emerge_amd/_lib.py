@@ -43,6 +43,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_pec_keep": (_c_int, [_p, _i64, _p, _p, _p]),
     "emg_field_points": (_c_int, [_p, _p, _i64, _p, _p, _p, _p]),
     "emg_farfield": (_c_int, [_p, _i64, _p, _p, _p, _p, _i64, _p, _p, ctypes.c_double, _p, _p]),
+    "emg_port_mode_elements": (_c_int, [_p, _i64, _p, _p, _p, ctypes.c_double, _p, _p]),
     "emg_farfield_noalloc": (_c_int, [_p, _i64, _p, _p, _p, _p, _i64, _p, _p, ctypes.c_double, _p, _p]),
     "emg_system_pattern_get": (_c_int, [_p, _p, _p, _pi64]),
     "emg_bc_add": (_c_int, [_p, _i64, _p, _p, ctypes.POINTER(ctypes.c_int32), _pi64]),

@@ -24,6 +24,7 @@ SOURCES = [
     "emg_farfield.hip",
     "emg_fields.hip",
     "emg_element.hip",
+    "emg_modal.hip",
 ]
 
 ARCH = os.environ.get("EMG_OFFLOAD_ARCH", "gfx950")

@@ -349,6 +349,20 @@ int emg_csr_solve(emg_ctx* ctx, int32_t nrhs, const double* b, int precision, do
 int emg_field_points(emg_ctx* ctx, const double* field, int64_t P, const double* pts, double* E, double* curlE,
                      int32_t* owner);
 
+/* Element matrices of the boundary-mode eigenproblem of a port face
+ * (replaces generalized_matrix_GQ + _matrix_builder,
+ * ASM/generalized_eigen_hb.py:247-458, called from assemble_bma_matrices,
+ * ASM/assembler.py:110-186): n triangles with local 2-D vertex coordinates
+ * xy (real [n][3][2], vertices in increasing global id), eps_r and mu_r^-1
+ * (complex128 [n][3][3], row-major), wavenumber k0.  A, B: complex128
+ * [n][14][14] in the mixed Nedelec-2 / Lagrange-2 order of
+ * E/elements/nedleg2.py:138-146 (edge x3, face, edge x3, face, vertex x3,
+ * edge x3), A = [Att - k0^2 Btt, 0; 0, 0], B = [Dtt, Dzt^T; Dzt, Dzz1 -
+ * k0^2 Dzz2] times |area| (the face-face block of Dtt is zero as in the
+ * reference).  Host buffers; synchronous. */
+int emg_port_mode_elements(emg_ctx* ctx, int64_t n, const double* xy, const double* eps, const double* inv_mu,
+                           double k0, double* A, double* B);
+
 /* Stratton-Chu far field (replaces stratton_chu_ff, MW/sc.py:41-132, called
  * by stratton_chu, :135-170): M surface samples with fields E, H (complex128
  * [3][M], interleaved), positions pos and area-weighted normals wn (real

@@ -1,0 +1,404 @@
+"""Oracle: boundary-mode (ModalPort) analysis of a port face.
+
+TEST INFRASTRUCTURE ONLY (see the package header): the checker of
+``emerge_amd.modal``; the product never imports it.
+
+Restates, per triangle and in plain NumPy:
+
+* the port surface mesh (``E/mesh3d.py:717-880``: sorted unique nodes, the
+  triangles renumbered, edges (i1,i2), (i2,i3), (i1,i3) of each triangle,
+  numbered lexicographically);
+* the mixed Nedelec-2 / Lagrange-2 DOF map (``E/elements/nedleg2.py:117-160``);
+* the 14x14 generalized-eigenproblem element matrices
+  (``ASM/generalized_eigen_hb.py:247-415``): basis functions of
+  ``:90-200`` evaluated with the 6-point rule of ``:236-241`` (its 8-digit
+  table), including the reference's omission of the face-face block of Dtt;
+* conductor DOF removal (``ASM/assembler.py:154-183``);
+* the dense LAPACK solve with the real-mode filter (``E/solver.py:192-227,
+  790-808``, sign -1, ermax = urmax = 2, minimum 1);
+* mode post-processing of ``MW/microwave_3d.py:583-652`` (phase of the
+  largest DOF, power flux of ``MW/port_functions.py:21-36`` with the E and H
+  interpolation of ``E/elements/ned2_interp.py:617-800``, TE/TM/TEM type).
+
+Pinning: ``tests/test_oracle_modal.py`` checks the rectangular-guide TE10
+propagation constant and mode shape against the closed form, TE20/TE01
+ordering, the TEM propagation constant k0 sqrt(eps_r) of a parallel-plate
+guide, the unit power of the normalised mode, and invariance of the
+eigenvalues under a rigid motion of the face.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg
+import scipy.sparse
+import scipy.sparse.linalg
+
+C0 = 299792458.0
+MU0 = 1.25663706212e-6
+
+# 6-point triangle rule of ASM/generalized_eigen_hb.py:236-241 (weights;
+# barycentric coordinates of the 3 vertices), 8 digits as tabulated there
+_W6 = np.array([0.22338159, 0.22338159, 0.22338159, 0.10995174, 0.10995174, 0.10995174])
+_L6 = np.array([[0.10810302, 0.44594849, 0.44594849, 0.81684757, 0.09157621, 0.09157621],
+                [0.44594849, 0.44594849, 0.10810302, 0.09157621, 0.09157621, 0.81684757],
+                [0.44594849, 0.10810302, 0.44594849, 0.09157621, 0.81684757, 0.09157621]])
+
+# local edges of a triangle with sorted vertices (E/mesh3d.py:873-878)
+EDGES = ((0, 1), (1, 2), (0, 2))
+
+
+def surface_mesh(nodes: np.ndarray, tris: np.ndarray, tri_ids: np.ndarray):
+    """(node ids (global), tris (3,n) local, edges (2,ne) local, tri_to_edge (3,n))."""
+    t = tris[:, tri_ids]
+    uniq = np.unique(t)
+    loc = {int(g): i for i, g in enumerate(uniq)}
+    lt = np.array([[loc[int(t[r, j])] for j in range(t.shape[1])] for r in range(3)])
+    # the reference numbers edges in set-iteration order; any numbering gives
+    # the same modes, this one is lexicographic
+    pairs = set()
+    for j in range(lt.shape[1]):
+        a, b, c = (int(x) for x in lt[:, j])
+        for e in ((a, b), (b, c), (a, c)):
+            pairs.add(tuple(sorted(e)))
+    emap = {e: i for i, e in enumerate(sorted(pairs))}
+    edges = np.array(sorted(pairs)).T
+    t2e = np.array([[emap[tuple(sorted((int(lt[p, j]), int(lt[q, j]))))] for j in range(lt.shape[1])]
+                    for p, q in EDGES])
+    return uniq, lt, edges, t2e
+
+
+def dof_map(n_nodes: int, edges: np.ndarray, tris: np.ndarray, t2e: np.ndarray):
+    """tri_to_field (14,n) and edge_to_field (5,ne), E/elements/nedleg2.py:117-160."""
+    ne, nt = edges.shape[1], tris.shape[1]
+    t2f = np.zeros((14, nt), dtype=np.int64)
+    t2f[0:3] = t2e
+    t2f[3] = np.arange(nt) + ne
+    t2f[4:7] = t2e + ne + nt
+    t2f[7] = np.arange(nt) + 2 * ne + nt
+    t2f[8:11] = tris + 2 * ne + 2 * nt
+    t2f[11:14] = t2e + 2 * ne + 2 * nt + n_nodes
+    e2f = np.zeros((5, ne), dtype=np.int64)
+    e2f[0] = np.arange(ne)
+    e2f[1] = np.arange(ne) + nt + ne
+    e2f[2:4] = edges + 2 * ne + 2 * nt
+    e2f[4] = np.arange(ne) + 2 * ne + 2 * nt + n_nodes
+    n_field = 2 * ne + 2 * nt + n_nodes + ne
+    return t2f, e2f, n_field
+
+
+def tri_coefficients(xs, ys):
+    """lam_i = (a_i + b_i x + c_i y) / (2A) with the orientation sign folded
+    in (ASM/generalized_eigen_hb.py:207-230); returns (a, b, c, A)."""
+    x1, x2, x3 = xs
+    y1, y2, y3 = ys
+    a = np.array([x2 * y3 - y2 * x3, x3 * y1 - y3 * x1, x1 * y2 - y1 * x2])
+    b = np.array([y2 - y3, y3 - y1, y1 - y2])
+    c = np.array([x3 - x2, x1 - x3, x2 - x1])
+    sA = 0.5 * ((x1 - x3) * (y2 - y1) - (x1 - x2) * (y3 - y1))
+    s = np.sign(sA)
+    return a * s, b * s, c * s, abs(sA)
+
+
+def _basis(xs, ys, px, py):
+    """Values at points (px, py) of the 8 transverse functions (8, 2, n),
+    their curls (8, n), the 6 Lagrange functions (6, n) and their gradients
+    (6, 2, n), in the reference's DOF order (edge-1 x3, face-1, edge-2 x3,
+    face-2; vertex x3, edge x3)."""
+    a, b, c, A = tri_coefficients(xs, ys)
+    a, b, c = a / (2 * A), b / (2 * A), c / (2 * A)
+    L = a[:, None] + b[:, None] * px[None] + c[:, None] * py[None]      # (3, n)
+    G = np.stack([b, c], axis=1)                                         # (3, 2) grad lam
+    cr = lambda i, j: b[i] * c[j] - c[i] * b[j]                         # grad lam_i x grad lam_j (z)
+
+    def w(i, j):  # lam_i grad lam_j - lam_j grad lam_i, and its curl 2 grad_i x grad_j
+        return L[i][None] * G[j][:, None] - L[j][None] * G[i][:, None], 2 * cr(i, j)
+
+    N = np.zeros((8, 2, px.size))
+    Cu = np.zeros((8, px.size))
+    for k, (p, q) in enumerate(EDGES):
+        wv, wc = w(p, q)
+        N[k] = wv
+        Cu[k] = wc
+        d = L[p] - L[q]
+        N[k + 4] = wv * d[None]
+        # curl(f w) = f curl w + grad f x w
+        gd = G[p] - G[q]
+        Cu[k + 4] = wc * d + (gd[0] * wv[1] - gd[1] * wv[0])
+    # face functions (ASM/generalized_eigen_hb.py:118-140)
+    w02, c02 = w(0, 2)
+    w12, c12 = w(1, 2)
+    w10, c10 = w(1, 0)
+    w20, c20 = w(2, 0)
+    N[3] = L[1][None] * w02 + L[0][None] * w12
+    Cu[3] = (L[1] * c02 + (G[1][0] * w02[1] - G[1][1] * w02[0])) + (L[0] * c12 + (G[0][0] * w12[1] - G[0][1] * w12[0]))
+    N[7] = L[2][None] * w10 + L[1][None] * w20
+    Cu[7] = (L[2] * c10 + (G[2][0] * w10[1] - G[2][1] * w10[0])) + (L[1] * c20 + (G[1][0] * w20[1] - G[1][1] * w20[0]))
+    P = np.zeros((6, px.size))
+    GP = np.zeros((6, 2, px.size))
+    for k in range(3):
+        P[k] = L[k] * (2 * L[k] - 1)
+        GP[k] = G[k][:, None] * (4 * L[k] - 1)[None]
+    for k, (p, q) in enumerate(EDGES):
+        P[k + 3] = 4 * L[p] * L[q]
+        GP[k + 3] = 4 * (G[p][:, None] * L[q][None] + G[q][:, None] * L[p][None])
+    return N, Cu, P, GP, A
+
+
+def element_matrices(xs, ys, Ms: np.ndarray, Mm: np.ndarray, k0: float):
+    """14x14 A, B of one triangle (ASM/generalized_eigen_hb.py:247-415): Ms =
+    mu_r^-1 (3x3), Mm = eps_r (3x3); A = [Att - k0^2 Btt, 0; 0, 0],
+    B = [Dtt, Dzt^T; Dzt, Dzz1 - k0^2 Dzz2], both times |area|.  The
+    reference leaves the face-face entries of Dtt at zero; so does this."""
+    px = _L6[0] * xs[0] + _L6[1] * xs[1] + _L6[2] * xs[2]
+    py = _L6[0] * ys[0] + _L6[1] * ys[1] + _L6[2] * ys[2]
+    N, Cu, P, GP, A = _basis(np.asarray(xs, float), np.asarray(ys, float), px, py)
+    W = _W6
+    msz, mmz = Ms[2, 2], Mm[2, 2]
+    Mst, Mmt = Ms[:2, :2], Mm[:2, :2]
+    q = lambda u, v: np.sum(W * u * v)                                   # scalar products
+    qv = lambda u, M, v: np.sum(W * np.einsum("an,ab,bn->n", u, M, v))  # u . (M v)
+    Att = np.array([[q(Cu[i], msz * Cu[j]) for j in range(8)] for i in range(8)], dtype=complex)
+    Btt = np.array([[qv(N[i], Mmt, N[j]) for j in range(8)] for i in range(8)], dtype=complex)
+    Dtt = np.array([[qv(N[i], Mst, N[j]) for j in range(8)] for i in range(8)], dtype=complex)
+    for i in (3, 7):
+        for j in (3, 7):
+            Dtt[i, j] = 0.0
+    # the reference fills face rows from face columns (symmetric copy)
+    for f in (3, 7):
+        for e in (0, 1, 2, 4, 5, 6):
+            Att[f, e] = Att[e, f]
+            Btt[f, e] = Btt[e, f]
+            Dtt[f, e] = Dtt[e, f]
+    Dzt = np.array([[qv(GP[k], Mst, N[j]) for j in range(8)] for k in range(6)], dtype=complex)
+    Dzz1 = np.array([[qv(GP[k], Mst, GP[l]) for l in range(6)] for k in range(6)], dtype=complex)
+    Dzz2 = np.array([[q(P[k], mmz * P[l]) for l in range(6)] for k in range(6)], dtype=complex)
+    Am = np.zeros((14, 14), dtype=complex)
+    Bm = np.zeros((14, 14), dtype=complex)
+    Am[:8, :8] = Att - k0 ** 2 * Btt
+    Bm[:8, :8] = Dtt
+    Bm[8:, :8] = Dzt
+    Bm[:8, 8:] = Dzt.T
+    Bm[8:, 8:] = Dzz1 - k0 ** 2 * Dzz2
+    return Am * A, Bm * A
+
+
+def assemble(xy: np.ndarray, tris: np.ndarray, t2f: np.ndarray, n_field: int, ur, er, k0):
+    """Global dense A, B (ASM/generalized_eigen_hb.py:420-458); ur, er (3,3,n)."""
+    A = np.zeros((n_field, n_field), dtype=complex)
+    B = np.zeros((n_field, n_field), dtype=complex)
+    for j in range(tris.shape[1]):
+        v = tris[:, j]
+        Ae, Be = element_matrices(xy[0, v], xy[1, v], np.linalg.inv(ur[:, :, j]), er[:, :, j], k0)
+        ids = t2f[:, j]
+        A[np.ix_(ids, ids)] += Ae
+        B[np.ix_(ids, ids)] += Be
+    return A, B
+
+
+def solve_modes(A, B, solve_ids, target_kz: float, direct: bool = True, nmodes: int = 6):
+    """Dense eig of the reduced pencil, real-mode filter (E/solver.py:192-227,
+    790-808): keeps 1 <= -lam <= 8 target_kz^2, ordered by |sqrt(-lam)|.
+    direct=False: ARPACK shift-invert about -target_kz^2 (E/solver.py:816-832)."""
+    As = A[np.ix_(solve_ids, solve_ids)]
+    Bs = B[np.ix_(solve_ids, solve_ids)]
+    if not direct:
+        return scipy.sparse.linalg.eigs(scipy.sparse.csc_matrix(As), k=nmodes, M=scipy.sparse.csc_matrix(Bs),
+                                        sigma=-(target_kz ** 2), which="LM")
+    lam, vec = scipy.linalg.eig(As, Bs)
+    ext = target_kz ** 2 * 2 * 2 * 2
+    mask = (-lam <= ext) & (-lam >= 1)
+    lam, vec = lam[mask], vec[:, mask]
+    order = np.argsort(np.abs(np.sqrt(-lam)))
+    return lam[order], vec[:, order]
+
+
+def interp_E(field, xy, tris, t2f, px, py):
+    """E (x, y, z local components) at local points (E/elements/ned2_interp.py:617-686)."""
+    out = np.zeros((3, px.size), dtype=complex)
+    for j in range(tris.shape[1]):
+        v = tris[:, j]
+        xs, ys = xy[0, v], xy[1, v]
+        a, b, c, A = tri_coefficients(xs, ys)
+        lam = (a[:, None] + b[:, None] * px[None] + c[:, None] * py[None]) / (2 * A)
+        inside = np.all(lam >= -1e-9, axis=0)
+        if not inside.any():
+            continue
+        N, _, P, _, _ = _basis(xs, ys, px[inside], py[inside])
+        e = field[t2f[:, j]]
+        out[0, inside] = np.einsum("i,in->n", e[:8], N[:, 0])
+        out[1, inside] = np.einsum("i,in->n", e[:8], N[:, 1])
+        out[2, inside] = np.einsum("i,in->n", e[8:], P)
+    return out
+
+
+def interp_H(field, xy, tris, t2f, px, py, k0, beta, urinv_diag):
+    """H at local points (E/elements/ned2_interp.py:688-800 with the factor
+    j / (k0 c0 mu0) of E/elements/nedleg2.py:180-188): the curl of
+    (E_t + z E_z) exp(-j beta z) times the diagonal of mu_r^-1."""
+    out = np.zeros((3, px.size), dtype=complex)
+    for j in range(tris.shape[1]):
+        v = tris[:, j]
+        xs, ys = xy[0, v], xy[1, v]
+        a, b, c, A = tri_coefficients(xs, ys)
+        lam = (a[:, None] + b[:, None] * px[None] + c[:, None] * py[None]) / (2 * A)
+        inside = np.all(lam >= -1e-9, axis=0)
+        if not inside.any():
+            continue
+        N, Cu, P, GP, _ = _basis(xs, ys, px[inside], py[inside])
+        e = field[t2f[:, j]]
+        Et = np.einsum("i,icn->cn", e[:8], N)
+        gz = np.einsum("i,icn->cn", e[8:], GP)
+        cz = np.einsum("i,in->n", e[:8], Cu)
+        # the reference's convention (ned2_interp.py:749-753): the mode
+        # varies as exp(+j beta z) along the local z axis
+        jb = 1j * beta
+        hx = gz[1] - jb * Et[1]
+        hy = -gz[0] + jb * Et[0]
+        hz = cz
+        f = 1j / (k0 * C0 * MU0)
+        d = urinv_diag[:, j]
+        out[0, inside] = f * hx * d[0]
+        out[1, inside] = f * hy * d[1]
+        out[2, inside] = f * hz * d[2]
+    return out
+
+
+def power_flux(field, xy, tris, t2f, k0, beta, urinv_diag):
+    """Time-averaged power through the face (MW/port_functions.py:21-36):
+    0.5 Re(E x H*) . z over the 6-point rule (full-precision Dunavant table)."""
+    W = np.array([0.223381589678011] * 3 + [0.109951743655322] * 3)
+    a_, b_ = (0.108103018168070, 0.445948490915965), (0.816847572980459, 0.091576213509771)
+    Lq = np.array([[a_[0], a_[1], a_[1]], [a_[1], a_[1], a_[0]], [a_[1], a_[0], a_[1]],
+                   [b_[0], b_[1], b_[1]], [b_[1], b_[1], b_[0]], [b_[1], b_[0], b_[1]]])
+    tot = 0.0
+    for j in range(tris.shape[1]):
+        v = tris[:, j]
+        xs, ys = xy[0, v], xy[1, v]
+        px, py = Lq @ xs, Lq @ ys
+        one = np.arange(3)
+        E = interp_E(field[t2f[:, j]], xy[:, v], one[:, None], np.arange(14)[:, None], px, py)
+        H = interp_H(field[t2f[:, j]], xy[:, v], one[:, None], np.arange(14)[:, None], px, py, k0, beta,
+                     urinv_diag[:, j:j + 1])
+        Sz = 0.5 * np.real(E[0] * np.conj(H[1]) - E[1] * np.conj(H[0]))
+        _, _, _, A = tri_coefficients(xs, ys)
+        tot += A * np.sum(W * Sz)
+    return tot
+
+
+def modal_analysis(nodes, tris, tri_ids, cs, er_tri, ur_tri, pec_tri_ids, k0, nmodes=6, tem=False, direct=True):
+    """Port modes of the face `tri_ids` (MW/microwave_3d.py:474-652 with
+    direct LAPACK eig): er_tri / ur_tri (3,3,n) per face triangle, conductor
+    edges from the triangles `pec_tri_ids` of the 3-D mesh.  Returns a list
+    of dicts (beta, field (n_field,), norm_factor, modetype, lam) in the
+    order the reference adds them, and the surface data."""
+    uniq, lt, edges, t2e = surface_mesh(nodes, tris, tri_ids)
+    t2f, e2f, n_field = dof_map(uniq.size, edges, lt, t2e)
+    x, y, _ = cs.in_local_cs(nodes[0, uniq], nodes[1, uniq], nodes[2, uniq])
+    xy = np.array([x, y])
+    A, B = assemble(xy, lt, t2f, n_field, ur_tri, er_tri, k0)
+    # conductor edges: every port edge that is an edge of a conductor triangle
+    gpos = {int(g): i for i, g in enumerate(uniq)}
+    emap = {(int(edges[0, i]), int(edges[1, i])): i for i in range(edges.shape[1])}
+    pec = set()
+    for t in pec_tri_ids:
+        v = [int(a) for a in tris[:, t]]
+        for p, q in ((v[0], v[1]), (v[1], v[2]), (v[0], v[2])):
+            if p in gpos and q in gpos:
+                k = tuple(sorted((gpos[p], gpos[q])))
+                if k in emap:
+                    pec.update(int(i) for i in e2f[:, emap[k]])
+    solve_ids = np.array([i for i in range(n_field) if i not in pec])
+    ermean = float(np.mean(er_tri[0, 0].real))
+    urmean = float(np.mean(ur_tri[0, 0].real))
+    target = ermean * urmean * (1.1 if tem else 0.7) * k0
+    lam, vec = solve_modes(A, B, solve_ids, target, direct, nmodes)
+    ermax = np.max(er_tri.real)
+    urmax = np.max(ur_tri.real)
+    urinv_diag = np.array([np.diag(np.linalg.inv(ur_tri[:, :, j])) for j in range(lt.shape[1])]).T
+    modes = []
+    first = None
+    for i in range(min(nmodes, lam.size)):
+        E = np.zeros(n_field, dtype=complex)
+        E[solve_ids] = vec[:, i]
+        E = E * np.exp(-1j * np.angle(E[np.argmax(np.abs(E))]))
+        beta = min(k0 * np.sqrt(ermax * urmax), np.emath.sqrt(-lam[i]))
+        energy = np.mean(np.abs(E) ** 2)
+        if energy < 1e-4:
+            continue
+        # MW/microwave_bc.py:693-700: after the first mode, the polarity is
+        # taken against the last mode of the first analysis -- the mode just
+        # appended when the analysis is the first one
+        polarity = 1.0
+        if first is not None:
+            polarity = float(np.sign(np.sum(E * E).real))
+        P = power_flux(E, xy, lt, t2f, k0, beta, urinv_diag)
+        ez = np.max(np.abs(E[2 * edges.shape[1] + 2 * lt.shape[1]:]))
+        exy = np.max(np.abs(E[:2 * edges.shape[1] + 2 * lt.shape[1]]))
+        mt = "TEM" if tem else ("TE" if ez / exy < 1e-1 else "TM")
+        modes.append(dict(beta=beta, field=E, P=P, polarity=polarity, modetype=mt, lam=lam[i]))
+        first = E
+    return modes, dict(uniq=uniq, tris=lt, edges=edges, t2f=t2f, xy=xy, n_field=n_field,
+                       solve_ids=solve_ids, A=A, B=B, urinv_diag=urinv_diag)
+
+
+class ModalPortOracle:
+    """Oracle-side port with the interface of ``oracle.ports`` built from
+    ``modal_analysis`` output (MW/microwave_bc.py:597-755): the fundamental
+    (largest beta) mode, E_function from ``interp_E`` in the port frame,
+    field = norm_factor * qmode(k0) * polarity * E, Uinc = -2j beta * field."""
+
+    def __init__(self, tri_ids, port_number, cs, modes, surf, freq, power=1.0, forced=None):
+        self.tri_ids = np.asarray(tri_ids)
+        self.port_number = port_number
+        self.cs = cs
+        self.power = power
+        self.forced = forced
+        self.surf = surf
+        self.freq = freq
+        order = sorted(range(len(modes)), key=lambda i: -np.real(modes[i]["beta"]))
+        self.mode = modes[order[0]]
+        self.first = modes[0]
+        k0 = self.mode_k0 = 2 * np.pi * freq / C0
+        q0 = self._qmode(k0, self.first)
+        self.norm = np.sqrt(1 / abs(self.mode["P"] * q0 ** 2))
+
+    active = False
+
+    def modetype(self, k0):
+        return self.mode["modetype"]
+
+    def Zmode(self, k0):
+        return self._qmode(k0) ** 2 * 376.73031366857
+
+    def get_beta(self, k0):
+        b = self.mode["beta"]
+        if self.forced == "TEM":
+            return b / self.mode_k0 * k0
+        f = k0 * C0 / (2 * np.pi)
+        return np.sqrt(b ** 2 + k0 ** 2 * (1 - (self.freq / f) ** 2))
+
+    def get_gamma(self, k0):
+        return 1j * self.get_beta(k0)
+
+    def _qmode(self, k0, mode=None):
+        mode = self.mode if mode is None else mode
+        Z0 = 376.73031366857
+        mt = mode["modetype"]
+        b = mode["beta"] if mode is not self.mode else self.get_beta(k0)
+        if mt == "TEM":
+            Zm = Z0
+        elif mt == "TE":
+            Zm = k0 * C0 / b * MU0
+        else:
+            Zm = b / (k0 * C0 * 8.854187818814e-12)
+        return np.sqrt(Zm / Z0)
+
+    def port_mode_3d_global(self, x, y, z, k0):
+        xl, yl, _ = self.cs.in_local_cs(x, y, z)
+        s = self.surf
+        E = interp_E(self.mode["field"], s["xy"], s["tris"], s["t2f"], np.atleast_1d(xl), np.atleast_1d(yl))
+        Eg = np.array(self.cs.in_global_basis(E[0], E[1], E[2]))
+        return self.norm * self._qmode(k0) * self.mode["polarity"] * Eg
+
+    def get_Uinc(self, x, y, z, k0):
+        return -2j * self.get_beta(k0) * self.port_mode_3d_global(x, y, z, k0)

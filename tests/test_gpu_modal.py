@@ -1,0 +1,103 @@
+"""ModalPort on the device against the boundary-mode oracle.
+
+Bar: the device element matrices within 1e-12 of the oracle's (relative to
+the largest entry); the port's propagation constant within 1e-10 and its
+normalised mode field within 1e-8 of the oracle's; a WR-90 sweep between two
+modal ports within |dS| <= 1e-6 of the oracle + SuperLU chain with the
+oracle's modal ports, and within the face discretisation (|dS| <= 5e-3) of
+the same sweep between analytic TE10 ports.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from emerge_amd import bc as pbc
+from emerge_amd._lib import ptr
+from emerge_amd.modal import ModalPort, modal_analysis
+from emerge_amd.sweep import FrequencySweep
+from emerge_amd.workloads import WR90_A, WR90_B
+from oracle import fem
+from oracle import modal as OM
+from oracle import ports as oports
+from tests import wg_case
+
+pytestmark = pytest.mark.gpu
+
+F0 = 10e9
+K0 = 2 * np.pi * F0 / OM.C0
+
+
+def test_element_matrices_match_oracle(hip_ctx):
+    g = np.random.default_rng(3)
+    n = 24
+    xy = g.uniform(-1e-2, 1e-2, (n, 3, 2))
+    def tens():
+        t = g.normal(size=(n, 3, 3)) + 1j * g.normal(size=(n, 3, 3)) * 0.1
+        return t + 3 * np.eye(3)[None]
+    er, ui = tens(), tens()
+    A = np.empty((n, 14, 14), dtype=np.complex128)
+    B = np.empty((n, 14, 14), dtype=np.complex128)
+    hip_ctx.call("emg_port_mode_elements", n, ptr(np.ascontiguousarray(xy)), ptr(np.ascontiguousarray(er)),
+                 ptr(np.ascontiguousarray(ui)), ctypes.c_double(K0), ptr(A), ptr(B))
+    for t in range(n):
+        Ao, Bo = OM.element_matrices(xy[t, :, 0], xy[t, :, 1], ui[t], er[t], K0)
+        assert np.abs(A[t] - Ao).max() <= 1e-12 * np.abs(Ao).max(), t
+        assert np.abs(B[t] - Bo).max() <= 1e-12 * np.abs(Bo).max(), t
+
+
+def _wr90_modal(c):
+    m = c["mesh"]
+    L = wg_case.L
+    t1, t2 = m.plane_tris(2, 0.0), m.plane_tris(2, L)
+    cs = [([1, 0, 0], [0, 1, 0], [0, 0, 1], [WR90_A / 2, WR90_B / 2, z]) for z in (0.0, L)]
+    return t1, t2, cs
+
+
+@pytest.fixture(scope="module")
+def modal_case(hip_ctx):
+    c = wg_case.case()
+    m, p = c["mesh"], c["problem"]
+    t1, t2, cs = _wr90_modal(c)
+    pec = pbc.PEC(c["pec"])
+    ports = [ModalPort(t, i + 1, pbc.CoordinateSystem(*cs[i])) for i, t in enumerate((t1, t2))]
+    for pt in ports:
+        modal_analysis(hip_ctx, m, pt, p.eps_r, p.inv_mu_r, [pec], F0)
+    eye = np.eye(3)[:, :, None] * np.ones(t1.size)
+    oracle_ports = []
+    for i, t in enumerate((t1, t2)):
+        ocs = oports.CS(*cs[i])
+        modes, surf = OM.modal_analysis(m.nodes, m.tris, t, ocs, eye.astype(complex), eye.astype(complex),
+                                        c["pec"], K0)
+        oracle_ports.append(OM.ModalPortOracle(t, i + 1, ocs, modes, surf, F0))
+    return c, ports, oracle_ports
+
+
+def test_modal_analysis_matches_oracle(modal_case):
+    c, ports, oports_ = modal_case
+    m = c["mesh"]
+    for pt, op in zip(ports, oports_):
+        assert pt.modetype(K0) == "TE" and op.modetype(K0) == "TE"
+        b, bo = pt.get_beta(K0), op.get_beta(K0)
+        assert abs(b / bo - 1) < 1e-10, (b, bo)
+        from emerge_amd.sweep import dunavant_points
+        P = dunavant_points(m.nodes, m.tris, pt.tri_ids).reshape(-1, 3).T
+        E = pt.port_mode_3d_global(P[0], P[1], P[2], K0)
+        Eo = op.port_mode_3d_global(P[0], P[1], P[2], K0)
+        assert np.abs(E - Eo).max() <= 1e-8 * np.abs(Eo).max()
+
+
+def test_sweep_with_modal_ports(modal_case):
+    c, ports, oports_ = modal_case
+    m, f, p = c["mesh"], c["basis"], c["problem"]
+    sw = FrequencySweep(m, f, p.eps_r, p.inv_mu_r, [pbc.PEC(c["pec"])] + ports)
+    freqs = [9e9, 10e9, 11e9]
+    r = sw.solve(freqs, precision="fp64", fblock=3)
+    assert np.all(r.relres <= 1e-8)
+    ra = FrequencySweep(m, f, p.eps_r, p.inv_mu_r, c["prod_bcs"]).solve(freqs, precision="fp64", fblock=3)
+    assert np.abs(r.S - ra.S).max() <= 5e-3, np.abs(r.S - ra.S).max()
+    # oracle + SuperLU with the oracle's modal ports at one frequency
+    K, b, pv, ids = fem.assemble_freq_matrix(m, c["maps"], c["E"], c["B"], c["pec"], oports_, freqs[1])
+    fields = fem.solve_all_ports(K, b, pv, ids)
+    So = fem.s_parameters(m, f.tet_to_field, fields, oports_, 2 * np.pi * freqs[1] / fem.C0)
+    assert np.abs(r.S[1] - So).max() <= 1e-6, np.abs(r.S[1] - So).max()
