@@ -101,3 +101,42 @@ def test_sweep_with_modal_ports(modal_case):
     fields = fem.solve_all_ports(K, b, pv, ids)
     So = fem.s_parameters(m, f.tet_to_field, fields, oports_, 2 * np.pi * freqs[1] / fem.C0)
     assert np.abs(r.S[1] - So).max() <= 1e-6, np.abs(r.S[1] - So).max()
+
+
+def test_parallel_plate_tem_line():
+    """A dielectric-filled parallel-plate line (conductors y = 0 and y = h,
+    open sides) between two TEM modal ports analysed at 6 GHz and swept at
+    5-7 GHz: |S21| = 1, |S11| ~ 0 and arg S21 = -k0 sqrt(eps_r) L (the TEM
+    beta scales with k0 away from the analysis frequency)."""
+    from emerge_amd._lib import Context
+    from emerge_amd.basis import Nedelec2
+    from emerge_amd.mesh import Mesh3D, kuhn_box
+    w, h, L, er = 8e-3, 2e-3, 20e-3, 2.2
+    nodes, tets = kuhn_box(6, 3, 12, size=(w, h, L))
+    m = Mesh3D(nodes, tets)
+    f = Nedelec2(m)
+    tv = m.nodes[:, m.tris]
+    on = lambda ax, val: np.nonzero(np.all(np.abs(tv[ax] - val) < 1e-12, axis=0))[0]
+    pec = pbc.PEC(np.concatenate([on(1, 0.0), on(1, h)]))
+    eps = np.full(m.n_tets, er, dtype=np.complex128)
+    imu = np.ones(m.n_tets, dtype=np.complex128)
+    ports = [ModalPort(on(2, z), i + 1, pbc.CoordinateSystem([1, 0, 0], [0, 1, 0], [0, 0, 1], [w / 2, h / 2, z]),
+                       modetype="TEM") for i, z in enumerate((0.0, L))]
+    ctx = Context(0)
+    try:
+        for pt in ports:
+            modal_analysis(ctx, m, pt, eps, imu, [pec], 6e9, TEM=True)
+            k = 2 * np.pi * 6e9 / OM.C0
+            assert abs(pt.get_beta(k) / (k * np.sqrt(er)) - 1) < 1e-9
+    finally:
+        ctx.close()
+    sw = FrequencySweep(m, f, eps, imu, [pec] + ports)
+    freqs = np.array([5e9, 6e9, 7e9])
+    r = sw.solve(freqs, precision="fp64", fblock=3)
+    assert np.all(r.relres <= 1e-8)
+    k0 = 2 * np.pi * freqs / OM.C0
+    s21 = r.S[:, 1, 0]
+    assert np.all(np.abs(np.abs(s21) - 1) < 2e-3), np.abs(s21)
+    assert np.all(np.abs(r.S[:, 0, 0]) < 2e-2), np.abs(r.S[:, 0, 0])
+    dphi = np.angle(s21 * np.exp(1j * k0 * np.sqrt(er) * L))
+    assert np.all(np.abs(dphi) < 1e-2), dphi
