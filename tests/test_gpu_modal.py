@@ -122,6 +122,10 @@ def test_parallel_plate_tem_line():
     imu = np.ones(m.n_tets, dtype=np.complex128)
     ports = [ModalPort(on(2, z), i + 1, pbc.CoordinateSystem([1, 0, 0], [0, 1, 0], [0, 0, 1], [w / 2, h / 2, z]),
                        modetype="TEM") for i, z in enumerate((0.0, L))]
+    # voltage lines from the lower to the upper plate (the mode polarity
+    # follows the voltage's sign): Z0 = eta0 h / (w sqrt(eps_r))
+    for pt, z in zip(ports, (0.0, L)):
+        pt.set_integration_line((w / 2, 0.0, z), (w / 2, h, z))
     ctx = Context(0)
     try:
         for pt in ports:
@@ -130,6 +134,9 @@ def test_parallel_plate_tem_line():
             assert abs(pt.get_beta(k) / (k * np.sqrt(er)) - 1) < 1e-9
     finally:
         ctx.close()
+    z0 = pbc.Z0 * h / (w * np.sqrt(er))
+    for pt in ports:
+        assert abs(pt.portZ0(k) / z0 - 1) < 1e-6, (pt.portZ0(k), z0)
     sw = FrequencySweep(m, f, eps, imu, [pec] + ports)
     freqs = np.array([5e9, 6e9, 7e9])
     r = sw.solve(freqs, precision="fp64", fblock=3)
