@@ -256,6 +256,43 @@ def config_bench(name: str, nf: int, fblock: int, precision: str = "fp64") -> di
                    relres_max=float(r.relres.max()),
                    s11_db={f"{f / 1e9:.2f}": round(float(20 * np.log10(abs(r.S[i, 0, 0]))), 3)
                            for i, f in enumerate(freqs)})
+    elif name == "modal":
+        # the WR-90 sweep of the bench line with modal ports: the TE10 mode
+        # of each port face from the boundary-mode analysis at 8 GHz
+        from emerge_amd.modal import ModalPort, modal_analysis
+        from emerge_amd import bc as hbc
+        p = w.waveguide()
+        ana = w.waveguide_bcs(p)
+        pec = [b for b in ana if isinstance(b, hbc.PEC)]
+        rw = [b for b in ana if getattr(b, "is_port", False)]
+        ports = [ModalPort(b.tri_ids, b.port_number, b.cs) for b in rw]
+        from emerge_amd._lib import Context
+        mctx = Context(0)
+        t0 = time.perf_counter()
+        for pt in ports:
+            modal_analysis(mctx, p.mesh, pt, p.eps_r, p.inv_mu_r, pec, 8e9)
+        t_modal = time.perf_counter() - t0
+        mctx.close()
+        freqs = np.linspace(8e9, 12e9, nf)
+        kws = dict(kw, keep_fields=False, fblock=32, restart=20000)
+        swm = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, pec + ports)
+        swa = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, ana)
+        swm.solve(freqs[:2], precision="mixed", **dict(kws, restart=50, max_cycles=1, fallback=False))
+        swa.solve(freqs[:2], precision="mixed", **dict(kws, restart=50, max_cycles=1, fallback=False))
+        t0 = time.perf_counter()
+        rm = swm.solve(freqs, precision="mixed", **kws)
+        tm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ra = swa.solve(freqs, precision="mixed", **kws)
+        ta = time.perf_counter() - t0
+        ok = int(np.sum(np.all(rm.relres <= 1e-8, axis=1)))
+        out.update(workload="WR-90 Kuhn 11x5x18 between two modal ports (boundary-mode analysis at 8 GHz), "
+                            f"{nf} points over 8-12 GHz, mixed batched F = 32",
+                   modal_analysis_s=round(t_modal, 3), beta_8GHz=[float(np.real(pt.get_beta(2 * np.pi * 8e9 / 299792458)))
+                                                                    for pt in ports],
+                   points=nf, converged=ok, seconds=round(tm, 3), freq_points_per_s=round(ok / tm, 2),
+                   analytic_ports_freq_points_per_s=round(nf / ta, 2),
+                   max_dS_vs_analytic_ports=float(np.abs(rm.S - ra.S).max()))
     elif name == "family":
         probs = w.waveguide_family()
         sws = [FrequencySweep(q.mesh, q.basis, q.eps_r, q.inv_mu_r, w.waveguide_bcs(q)) for q in probs]
@@ -457,7 +494,7 @@ def main() -> int:
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=None,
                     help="GMRES dimension / IDR matvecs per cycle / COCG iterations per cycle")
-    ap.add_argument("--config", choices=["cavity", "patch", "family", "farfield", "element"], default=None,
+    ap.add_argument("--config", choices=["cavity", "patch", "family", "farfield", "element", "modal"], default=None,
                     help="run one full-size solve workload instead of the default bench")
     ap.add_argument("--config-points", type=int, default=None, help="frequency points of --config")
     ap.add_argument("--config-precision", default="fp64", choices=["fp64", "mixed", "m32"],
@@ -500,7 +537,8 @@ def main() -> int:
     from emerge_amd.workloads import cube_assembly
 
     if args.config:
-        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "farfield": 0, "element": 0}[args.config]
+        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "farfield": 0, "element": 0,
+                                    "modal": 201}[args.config]
         fb = 8 if args.fblock == "auto" else args.fblock
         print(json.dumps(config_bench(args.config, nf, fb if args.config == "patch" else 32, args.config_precision)),
               flush=True)
