@@ -274,7 +274,7 @@ def config_bench(name: str, nf: int, fblock: int, precision: str = "fp64") -> di
         t_modal = time.perf_counter() - t0
         mctx.close()
         freqs = np.linspace(8e9, 12e9, nf)
-        kws = dict(kw, keep_fields=False, fblock=32, restart=20000)
+        kws = dict(kw, keep_fields=False, fblock=64, restart=20000)
         swm = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, pec + ports)
         swa = FrequencySweep(p.mesh, p.basis, p.eps_r, p.inv_mu_r, ana)
         swm.solve(freqs[:2], precision="mixed", **dict(kws, restart=50, max_cycles=1, fallback=False))
@@ -287,7 +287,7 @@ def config_bench(name: str, nf: int, fblock: int, precision: str = "fp64") -> di
         ta = time.perf_counter() - t0
         ok = int(np.sum(np.all(rm.relres <= 1e-8, axis=1)))
         out.update(workload="WR-90 Kuhn 11x5x18 between two modal ports (boundary-mode analysis at 8 GHz), "
-                            f"{nf} points over 8-12 GHz, mixed batched F = 32",
+                            f"{nf} points over 8-12 GHz, mixed batched F = 64",
                    modal_analysis_s=round(t_modal, 3), beta_8GHz=[float(np.real(pt.get_beta(2 * np.pi * 8e9 / 299792458)))
                                                                     for pt in ports],
                    points=nf, converged=ok, seconds=round(tm, 3), freq_points_per_s=round(ok / tm, 2),
@@ -487,7 +487,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-plain", type=int, default=4, help="FP64 plain-path frequency points")
     ap.add_argument("--sweep-mixed", type=int, default=201, help="mixed batched frequency points")
-    ap.add_argument("--fblock", default="32", help="systems per mixed batch (1..32) or 'auto' (tuned per mesh)")
+    ap.add_argument("--fblock", default="64", help="systems per batch (1..64, 64 with COCG only) or 'auto' "
+                                                   "(tuned per mesh)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-only", action="store_true", help="profile helper: run only the solve sweep")
     ap.add_argument("--method", default="cocg", choices=["gmres", "idr", "cocg"])

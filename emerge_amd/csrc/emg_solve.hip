@@ -1876,7 +1876,11 @@ int dispatch_batch(emg_ctx* c, int F, const Op<R>& op, const Op<double>& op64, c
         case 8: return run_batch<R, 8>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
         case 16: return run_batch<R, 16>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
         case 32: return run_batch<R, 32>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
-        default: return fail(c, EMG_ERR_ARG, "batch width %d not in {1,2,4,8,16,32}", F);
+        case 64:
+            // the widest block: COCG only (GMRES and IDR keep per-system bases)
+            if (P.method != 2) return fail(c, EMG_ERR_ARG, "batch width 64 needs COCG");
+            return run_cocg<R, 64>(c, op, op64, k2, gam, nbc, b64, x64, P, st);
+        default: return fail(c, EMG_ERR_ARG, "batch width %d not in {1,2,4,8,16,32,64}", F);
     }
 #undef run_batch
 }
@@ -2075,7 +2079,7 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     if (precision < 0 || precision > 2) return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision must be 0, 1 or 2");
     if (precision == 2 && c->krylov_method != 2)
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision 2 (complex64 mass values) needs COCG");
-    fblock = std::max(1, std::min(fblock, 32));
+    fblock = std::max(1, std::min(fblock, c->krylov_method == 2 ? 64 : 32));
     hipStream_t s = c->stream;
     const int64_t n = c->n_red;
     const int nbc = c->n_bc, np = c->n_forcing;

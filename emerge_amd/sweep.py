@@ -713,7 +713,7 @@ class FrequencySweep:
         return [0] + [g for g in (4, 8, 16, 32) if v <= g <= gw]
 
     def autotune(self, freqs, precision: str = "mixed", method: str = "cocg", precond: str = "aux",
-                 candidates=(4, 8, 16, 32), iters: int = 200, idr_s: int = 4, inner_rel32: float = 1e-2,
+                 candidates=(4, 8, 16, 32, 64), iters: int = 200, idr_s: int = 4, inner_rel32: float = 1e-2,
                  lanes: bool = True) -> tuple:
         """Frequency-block width F and SpMV lane layout (entries of a row in
         flight, hence rows per wavefront) with the least device time per
@@ -741,6 +741,8 @@ class FrequencySweep:
         self.ctx.call("emg_solver_set_escalation", 0)
         try:
             for F in candidates:
+                if F > 32 and method != "cocg":
+                    continue  # blocks of 64 are COCG only
                 opts = self.spmv_lane_options(F, precision, safe) if (lanes and method == "cocg") else [0]
                 rows = np.arange(F) % nb
                 run = dict(fblock=F, tol=1e-300, restart=iters if method != "gmres" else min(iters, 100),
@@ -786,7 +788,7 @@ class FrequencySweep:
         (``aux_space``), 'jacobi' = Jacobi alone.  fallback (mixed only):
         systems that miss tol are solved again in FP64
         (``SweepResult.fp64_fallback`` marks them).  fblock = systems per
-        device batch (1..32) or 'auto' (``autotune_fblock``).  safe32
+        device batch (1..32; up to 64 with COCG) or 'auto' (``autotune_fblock``).  safe32
         (mixed COCG): keep the real stiffness in FP64 and accumulate the
         SpMV and the gradient-space correction in FP64; 'auto' turns it on
         when ``needs_safe32`` says the mesh is fine against the wavelength.

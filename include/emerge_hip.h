@@ -178,8 +178,8 @@ int emg_forcing_get(emg_ctx* ctx, double* out);
  * complex128 inner solves from their iterate: emg_sweep_escalated), 2 (COCG
  * only) = complex128 inner solves whose SpMV reads the symmetric mass part
  * as complex64 inside the same FP64 refinement; the
- * method is emg_solver_config's; fblock = max systems per batch (1..32,
- * matrix read once per batch).  x_out complex [nbatch][n_red]; relres_out =
+ * method is emg_solver_config's; fblock = max systems per batch (1..32, up
+ * to 64 with COCG; the matrix is read once per batch).  x_out complex [nbatch][n_red]; relres_out =
  * final FP64 ||b - A x|| / ||b||. */
 int emg_sweep_solve(emg_ctx* ctx, int64_t nbatch, const double* k2, const double* gamma,
                     const double* rhs_coef, int precision, int fblock, double tol, int restart,

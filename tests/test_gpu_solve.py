@@ -425,3 +425,14 @@ def test_solver_plugin_with_attached_mesh(precision):
         # a second right-hand side on the same matrix reuses the device state
         x2, rep2 = att.solve(A, 2 * b, None, True, 1)
         assert np.linalg.norm(x2 - 2 * x1) <= 1e-6 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_cocg_block_of_64(sweep, precision):
+    """64 (frequency, port) systems in one COCG block: every residual <= 1e-8
+    and S equal to the same systems solved in blocks of 32."""
+    freqs = np.linspace(8e9, 12e9, 32)
+    r64 = sweep.solve(freqs, precision=precision, fblock=64)
+    r32 = sweep.solve(freqs, precision=precision, fblock=32)
+    assert np.all(r64.relres <= 1e-8) and np.all(r32.relres <= 1e-8)
+    assert np.abs(r64.S - r32.S).max() <= 1e-6
