@@ -21,9 +21,10 @@ Restates, per triangle and in plain NumPy:
   interpolation of ``E/elements/ned2_interp.py:617-800``, TE/TM/TEM type).
 
 Pinning: ``tests/test_oracle_modal.py`` checks the rectangular-guide TE10
-propagation constant and mode shape against the closed form, TE20/TE01
-ordering, the TEM propagation constant k0 sqrt(eps_r) of a parallel-plate
-guide, the unit power of the normalised mode, and invariance of the
+propagation constant (converging under refinement, TE20/TE01 cut off) and
+its unit-power field against the closed form, the TEM propagation constant
+k0 sqrt(eps_r) of a parallel-plate guide, the slab-loaded parallel plate's
+characteristic-equation root (inhomogeneous filling), and invariance of the
 eigenvalues under a rigid motion of the face.
 """
 from __future__ import annotations

@@ -147,3 +147,22 @@ def test_parallel_plate_tem_line():
     assert np.all(np.abs(r.S[:, 0, 0]) < 2e-2), np.abs(r.S[:, 0, 0])
     dphi = np.angle(s21 * np.exp(1j * k0 * np.sqrt(er) * L))
     assert np.all(np.abs(dphi) < 1e-2), dphi
+
+
+def test_slab_loaded_parallel_plate_matches_oracle(hip_ctx):
+    """Inhomogeneous port face: the device pencil's fundamental mode equals
+    the oracle's (1e-10) and the slab-loaded closed form (1e-3)."""
+    from tests.test_oracle_modal import _slab_case
+    c = _slab_case()
+    m, port = c["mesh"], c["port"]
+    f = 10e9
+    pt = ModalPort(port, 1, pbc.CoordinateSystem(*c["cs"]))
+    modal_analysis(hip_ctx, m, pt, c["eps_t"].astype(np.complex128), np.ones(m.n_tets, dtype=np.complex128),
+                   [pbc.PEC(c["pec"])], f)
+    er = (np.eye(3)[:, :, None] * c["eps_t"][m.tri_to_tet[0, port]][None, None, :]).astype(complex)
+    ur = (np.eye(3)[:, :, None] * np.ones(port.size)).astype(complex)
+    modes, _ = OM.modal_analysis(m.nodes, m.tris, port, oports.CS(*c["cs"]), er, ur, c["pec"], c["k0"])
+    bo = max(np.real(md["beta"]) for md in modes)
+    b = np.real(pt.get_beta(c["k0"]))
+    assert abs(b / bo - 1) < 1e-10, (b, bo)
+    assert abs(b / c["exact"][0] - 1) < 1e-3

@@ -106,3 +106,45 @@ def test_rigid_motion_invariance(wr90_coarse):
     m, port, pec, cs, eye = _wr90(11, 5, rot=rot, shift=np.array([0.3, -1.2, 2.0]))
     moved, _ = OM.modal_analysis(m.nodes, m.tris, port, cs, eye, eye, pec, K0)
     assert abs(moved[0]["beta"] / modes[0]["beta"] - 1) < 1e-9
+
+
+def _slab_case():
+    """Parallel plates (conductors y = 0, y = h, open sides) loaded with a
+    dielectric slab eps1 on x < a and eps2 beyond: the E_y(x) mode's beta
+    solves kx1 tan(kx1 a) = ka tanh(ka (W - a)), kx1^2 = k0^2 eps1 - beta^2,
+    ka^2 = beta^2 - k0^2 eps2 (magnetic side walls)."""
+    from scipy.optimize import brentq
+    W, hh, a, e1, e2 = 8e-3, 2e-3, 3e-3, 4.0, 1.0
+    k0 = 2 * np.pi * 10e9 / OM.C0
+
+    def char(beta):
+        kx1 = np.sqrt(k0 ** 2 * e1 - beta ** 2)
+        ka = np.sqrt(beta ** 2 - k0 ** 2 * e2)
+        return kx1 * np.tan(kx1 * a) - ka * np.tanh(ka * (W - a))
+
+    bs = np.linspace(k0 * np.sqrt(e2) * 1.0001, k0 * np.sqrt(e1) * 0.9999, 2000)
+    v = char(bs)
+    roots = [brentq(char, bs[i], bs[i + 1]) for i in range(len(bs) - 1)
+             if np.sign(v[i]) != np.sign(v[i + 1]) and abs(v[i]) < 1e3 and abs(v[i + 1]) < 1e3]
+    nodes, tets = kuhn_box(16, 2, 1, size=(W, hh, 1e-3))
+    m = Mesh3D(nodes, tets)
+    tv = m.nodes[:, m.tris]
+    on = lambda ax, val: np.all(np.abs(tv[ax] - val) < 1e-12, axis=0)
+    port = np.nonzero(on(2, 0.0))[0]
+    pec = np.nonzero(on(1, 0.0) | on(1, hh))[0]
+    eps_t = np.where(m.nodes[:, m.tets].mean(axis=1)[0] < a, e1, e2)
+    cs = ([1, 0, 0], [0, 1, 0], [0, 0, 1], [W / 2, hh / 2, 0])
+    return dict(mesh=m, port=port, pec=pec, eps_t=eps_t, cs=cs, k0=k0, exact=np.array(roots))
+
+
+def test_slab_loaded_parallel_plate():
+    """Inhomogeneous filling: the face's fundamental mode against the root
+    of the slab-loaded parallel-plate characteristic equation."""
+    c = _slab_case()
+    assert c["exact"].size == 1
+    m, port = c["mesh"], c["port"]
+    er = (np.eye(3)[:, :, None] * c["eps_t"][m.tri_to_tet[0, port]][None, None, :]).astype(complex)
+    ur = (np.eye(3)[:, :, None] * np.ones(port.size)).astype(complex)
+    modes, _ = OM.modal_analysis(m.nodes, m.tris, port, oports.CS(*c["cs"]), er, ur, c["pec"], c["k0"])
+    beta = max(np.real(md["beta"]) for md in modes)
+    assert abs(beta / c["exact"][0] - 1) < 1e-3, (beta / c["k0"], c["exact"] / c["k0"])
