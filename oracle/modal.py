@@ -34,8 +34,10 @@ import scipy.linalg
 import scipy.sparse
 import scipy.sparse.linalg
 
+# E/const.py:2-6
 C0 = 299792458.0
-MU0 = 1.25663706212e-6
+EPS0 = 8.854187818814e-12
+MU0 = 1 / (C0 * C0 * EPS0)
 
 # 6-point triangle rule of ASM/generalized_eigen_hb.py:236-241 (weights;
 # barycentric coordinates of the 3 vertices), 8 digits as tabulated there
@@ -391,7 +393,7 @@ class ModalPortOracle:
         elif mt == "TE":
             Zm = k0 * C0 / b * MU0
         else:
-            Zm = b / (k0 * C0 * 8.854187818814e-12)
+            Zm = b / (k0 * C0 * EPS0)
         return np.sqrt(Zm / Z0)
 
     def port_mode_3d_global(self, x, y, z, k0):
