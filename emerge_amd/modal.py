@@ -79,8 +79,7 @@ class PortFace:
         self.edge_to_field = e2f
         x, y, _ = cs.in_local_cs(*mesh.nodes[:, self.node_ids])
         self.xy = np.array([x, y])
-        self._edge_key = {int(k): i for i, k in enumerate(ek)}
-        self._gid = {int(g): i for i, g in enumerate(self.node_ids)}
+        self._ek = ek                                      # sorted edge keys: edge e has key ek[e]
         # barycentric coefficients per triangle: lam_i = a_i + b_i x + c_i y
         v = self.xy[:, self.tris]                          # (2, 3, nt)
         x1, x2, x3 = v[0]
@@ -92,18 +91,22 @@ class PortFace:
         self.b = np.array([y2 - y3, y3 - y1, y1 - y2]) * s
         self.c = np.array([x3 - x2, x1 - x3, x2 - x1]) * s
 
+    def shared_edges(self, tris3d: np.ndarray) -> np.ndarray:
+        """Face edges that are edges of the given 3-D triangles (3, k)."""
+        nn = self.node_ids.size
+        pos = np.minimum(np.searchsorted(self.node_ids, tris3d), nn - 1)
+        hit = self.node_ids[pos] == tris3d
+        found = []
+        for p, q in ((0, 1), (1, 2), (0, 2)):
+            ok = hit[p] & hit[q]
+            keys = np.minimum(pos[p][ok], pos[q][ok]) * nn + np.maximum(pos[p][ok], pos[q][ok])
+            i = np.minimum(np.searchsorted(self._ek, keys), self._ek.size - 1)
+            found.append(i[self._ek[i] == keys])   # edge e has key _ek[e]
+        return np.unique(np.concatenate(found))
+
     def vertex_xy(self) -> np.ndarray:
         """(nt, 3, 2) local vertex coordinates, vertices in increasing id."""
         return np.ascontiguousarray(np.moveaxis(self.xy[:, self.tris], 0, 2).transpose(1, 0, 2))
-
-    def edge_dofs(self, gi: int, gj: int):
-        """DOFs of the face edge between global nodes gi, gj (None if absent)."""
-        a, b = self._gid.get(int(gi)), self._gid.get(int(gj))
-        if a is None or b is None:
-            return None
-        a, b = min(a, b), max(a, b)
-        e = self._edge_key.get(a * self.node_ids.size + b)
-        return None if e is None else self.edge_to_field[:, e]
 
     # ------------------------------------------------------------ evaluation
     def locate(self, x, y):
@@ -369,12 +372,8 @@ def modal_analysis(ctx, mesh, port: ModalPort, eps_r, inv_mu_r=None, boundary_co
         pec.update(int(i) for i in face.tri_to_field[:, it])
     for b in boundary_conditions:
         if isinstance(b, bcs.PEC) and len(b.tri_ids):
-            tv = mesh.tris[:, b.tri_ids]
-            for p, q in ((0, 1), (1, 2), (0, 2)):
-                for gi, gj in zip(tv[p], tv[q]):
-                    d = face.edge_dofs(gi, gj)
-                    if d is not None:
-                        pec.update(int(i) for i in d)
+            for e in face.shared_edges(mesh.tris[:, b.tri_ids]):
+                pec.update(int(i) for i in face.edge_to_field[:, e])
     solve_ids = np.array(sorted(set(range(n)) - pec), dtype=np.int64)
 
     ermean = float(np.mean(er[0, 0].real))

@@ -46,3 +46,16 @@ def test_basis_fields_and_power_match_oracle():
     P = face.power_flux(field, k0, beta, urinv)
     Po = OM.power_flux(field, face.xy, face.tris, face.tri_to_field, k0, beta, urinv)
     assert abs(P - Po) <= 1e-12 * abs(Po)
+
+
+def test_shared_edges():
+    """Face edges shared with 3-D triangles: all of them for the face's own
+    triangles; the face's boundary edges on the walls y = 0 and y = h."""
+    m, port, cs = _face()
+    face = PortFace(m, port, pbc.CoordinateSystem(*cs))
+    assert np.array_equal(face.shared_edges(m.tris[:, port]), np.arange(face.edges.shape[1]))
+    tv = m.nodes[:, m.tris]
+    walls = np.nonzero(np.all(np.abs(tv[1]) < 1e-12, axis=0) | np.all(np.abs(tv[1] - 3e-3) < 1e-12, axis=0))[0]
+    e = face.shared_edges(m.tris[:, walls])
+    y = face.xy[1][face.edges[:, e]]
+    assert e.size == 2 * 5 and np.all(np.abs(np.abs(y) - 1.5e-3) < 1e-12)
