@@ -11,9 +11,10 @@ reference's ``SolverLAPACK``) or ARPACK shift-invert (``direct=False``, as
 analytic port mode: ``get_gamma``, ``uinc_shape`` / ``uinc_factor`` for the
 forcing and ``mode_shape`` / ``mode_factor`` for the S-parameter overlaps.
 
-Not carried over: alignment vectors (``align_modes``), automatic TEM
-conductor detection (a TEM port's impedance needs ``set_integration_line``),
-and per-frequency re-analysis of ports with mixed materials.
+Not carried over: automatic TEM conductor detection (a TEM port's impedance
+needs ``set_integration_line``) and per-frequency re-analysis of ports with
+mixed materials.  Alignment vectors sample the face at its triangle centres
+(the reference samples a 5 x 5 grid of the face's gmsh parametrisation).
 """
 from __future__ import annotations
 
@@ -240,6 +241,7 @@ class ModalPort(bcs.PortBC):
         self._first_k0 = None
         self._last_k0 = None
         self.vintline: list[np.ndarray] = []
+        self.alignment_vectors: list[np.ndarray] = []
         self.initialized = False
 
     # mode bookkeeping (MW/microwave_bc.py:597-710)
@@ -309,9 +311,34 @@ class ModalPort(bcs.PortBC):
             mode.polarity = float(np.sign(np.sum(mode.modefield * ref).real))
         return mode
 
+    def align_modes(self, *axes) -> None:
+        """Mode order and polarity by alignment vectors (MW/microwave_bc.py:561-575):
+        mode i is the one whose field has the largest |sum E . axes[i]| over
+        the face samples, with that sum made positive."""
+        self.alignment_vectors = [np.asarray(a, dtype=float) for a in axes]
+
+    def _face_samples(self, mode: PortMode):
+        """Sample points of the face: its triangle centroids (global)."""
+        f = mode.face
+        cx, cy = f.xy[0, f.tris].mean(axis=0), f.xy[1, f.tris].mean(axis=0)
+        o, ex, ey = f.cs.origin, f.cs.xax, f.cs.yax
+        return [o[k] + ex[k] * cx + ey[k] * cy for k in range(3)]
+
     def sort_modes(self) -> None:
-        for k0 in self.modes:
-            self.modes[k0] = sorted(self.modes[k0], key=lambda m: np.real(m.beta), reverse=True)
+        """MW/microwave_bc.py:614-639: by the alignment vectors when given,
+        else by decreasing beta."""
+        for k0, modes in self.modes.items():
+            if self.alignment_vectors:
+                x, y, z = self._face_samples(modes[0])
+                picked = []
+                for ax in self.alignment_vectors:
+                    ints = [np.sum(np.einsum("cn,c->n", m.E_global(x, y, z), ax)) for m in modes]
+                    i = int(np.argmax(np.abs(ints)))
+                    modes[i].polarity = float(np.sign(np.real(ints[i])))
+                    picked.append(modes[i])
+                self.modes[k0] = picked
+            else:
+                self.modes[k0] = sorted(modes, key=lambda m: np.real(m.beta), reverse=True)
 
 
 def _face_frame(mesh, tri_ids) -> bcs.CoordinateSystem:
@@ -406,6 +433,13 @@ def modal_analysis(ctx, mesh, port: ModalPort, eps_r, inv_mu_r=None, boundary_co
         E[solve_ids] = vec[:, i]
         E = E * np.exp(-1j * np.angle(E[np.argmax(np.abs(E))]))
         beta = min(k0 * np.sqrt(ermax * urmax), np.emath.sqrt(-lam[i]))
+        if i < len(port.alignment_vectors):
+            # MW/microwave_3d.py:595-602: the mean field over the triangle
+            # centres points along alignment vector i
+            probe = PortMode(face, E, k0, beta, freq, urinv_diag)
+            Ec = probe.E_global(*port._face_samples(probe)).mean(axis=1)
+            if np.real(np.dot(port.alignment_vectors[i], Ec)) < 0:
+                E = -E
         P = face.power_flux(E, k0, beta, urinv_diag)
         mode = port.add_mode(PortMode(face, E, k0, beta, freq, urinv_diag))
         if mode is None:

@@ -166,3 +166,28 @@ def test_slab_loaded_parallel_plate_matches_oracle(hip_ctx):
     b = np.real(pt.get_beta(c["k0"]))
     assert abs(b / bo - 1) < 1e-10, (b, bo)
     assert abs(b / c["exact"][0] - 1) < 1e-3
+
+
+def test_align_modes_picks_te10_and_te01(hip_ctx):
+    """At 16 GHz the WR-90 face carries TE10, TE20 and TE01: alignment
+    vectors y and x select TE10 (E_y with nonzero mean) and TE01 (E_x), each
+    with a positive mean along its vector."""
+    c = wg_case.case()
+    m, p = c["mesh"], c["problem"]
+    t1, _, cs = _wr90_modal(c)
+    pt = ModalPort(t1, 1, pbc.CoordinateSystem(*cs[0]))
+    pt.align_modes((0, 1, 0), (1, 0, 0))
+    modal_analysis(hip_ctx, m, pt, p.eps_r, p.inv_mu_r, [pbc.PEC(c["pec"])], 16e9)
+    k0 = 2 * np.pi * 16e9 / OM.C0
+    te10 = np.sqrt(k0 ** 2 - (np.pi / WR90_A) ** 2)
+    te01 = np.sqrt(k0 ** 2 - (np.pi / WR90_B) ** 2)
+    assert abs(np.real(pt.get_mode(k0, 0).beta) / te10 - 1) < 3e-3
+    # TE01 is near cut-off at 16 GHz: kc^2 / beta^2 = 5.7 amplifies the face
+    # discretisation error of its cut-off wavenumber (TE20 would be 50% off)
+    assert abs(np.real(pt.get_mode(k0, 1).beta) / te01 - 1) < 2e-2
+    g = np.random.default_rng(2)
+    x, y = g.uniform(0.1, 0.9, 50) * WR90_A, g.uniform(0.1, 0.9, 50) * WR90_B
+    z = np.zeros_like(x)
+    assert np.mean(np.real(pt.mode_shape(x, y, z)[1])) > 0
+    pt.selected_mode = 1
+    assert np.mean(np.real(pt.mode_shape(x, y, z)[0])) > 0
