@@ -342,13 +342,22 @@ class ModalPort(bcs.PortBC):
 
 
 def _face_frame(mesh, tri_ids) -> bcs.CoordinateSystem:
-    v = mesh.nodes[:, mesh.tris[:, tri_ids]]
-    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], axis=0).sum(axis=1)
-    z = n / np.linalg.norm(n)
-    e = v[:, 1, 0] - v[:, 0, 0]
-    x = e - np.dot(e, z) * z
-    x = x / np.linalg.norm(x)
-    return bcs.CoordinateSystem(x, np.cross(z, x), z, v.reshape(3, -1).mean(axis=1))
+    """Frame of a port given without one: z along the face's normal pointing
+    out of the volume (away from the tet behind each triangle), x and y as
+    Axis.construct_cs builds them (E/cs.py:106-118: x-axis unless z is within
+    0.999 of it, then y), origin at the face's centroid (the origin only
+    shifts the local coordinates the modes are solved and evaluated in)."""
+    v = mesh.nodes[:, mesh.tris[:, tri_ids]]                     # (3, 3, n)
+    nt = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], axis=0)  # (3, n)
+    tet = mesh.tets[:, mesh.tri_to_tet[0, tri_ids]]              # (4, n)
+    inside = mesh.nodes[:, tet].mean(axis=1) - v.mean(axis=1)    # centroid of the tet minus the triangle's
+    nt = nt * -np.sign(np.einsum("in,in->n", nt, inside))
+    z = nt.sum(axis=1)
+    z = z / np.linalg.norm(z)
+    ax = np.array([1.0, 0, 0]) if abs(z[0]) <= 0.999 else np.array([0, 1.0, 0])
+    y = np.cross(z, ax)
+    x = -np.cross(z, y)
+    return bcs.CoordinateSystem(x, y, z, v.reshape(3, -1).mean(axis=1))
 
 
 def _per_tri_tensor(values, tets) -> np.ndarray:

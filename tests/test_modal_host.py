@@ -59,3 +59,17 @@ def test_shared_edges():
     e = face.shared_edges(m.tris[:, walls])
     y = face.xy[1][face.edges[:, e]]
     assert e.size == 2 * 5 and np.all(np.abs(np.abs(y) - 1.5e-3) < 1e-12)
+
+
+def test_face_frame_without_cs():
+    """A port without a frame gets one from its face: orthonormal, z along
+    the normal out of the volume (-z for the face z = 0 of the box, although
+    its triangles are wound both ways), origin at the face's centroid."""
+    from emerge_amd.modal import _face_frame
+    m, port, _ = _face()
+    cs = _face_frame(m, port)
+    B = np.array([cs.xax, cs.yax, cs.zax])
+    assert np.allclose(B @ B.T, np.eye(3), atol=1e-12)
+    assert np.allclose(cs.zax, [0, 0, -1], atol=1e-12)
+    assert np.allclose(np.cross(cs.xax, cs.yax), cs.zax, atol=1e-12)
+    assert np.allclose(cs.origin, [3.5e-3, 1.5e-3, 0.0], atol=1e-12)
