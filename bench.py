@@ -320,23 +320,29 @@ def config_bench(name: str, nf: int, fblock: int, precision: str = "fp64") -> di
         for sw in sws:
             sw.ctx.close()
         # the same meshes as one block-diagonal system: one launch per kernel
+        # for all meshes, with per-mesh Krylov scalars (row segments) and,
+        # for comparison, with scalars global over the union
         from emerge_amd.family import FamilySweep
-        fam = FamilySweep(probs, [w.waveguide_bcs(q) for q in probs])
-        kfam = dict(kf)
-        fam.solve(freqs[:2], precision="mixed", **kfam)
-        t0 = time.perf_counter()
-        Sf, bound, rf = fam.solve(freqs, precision="mixed", **kfam)
-        tf = time.perf_counter() - t0
-        ok_f = int(np.sum(np.all(bound <= 1e-8, axis=1))) * len(probs)
-        union = {"seconds": round(tf, 3), "converged_points": ok_f, "freq_points_per_s": round(ok_f / tf, 3),
-                 "reduced_dofs": int(fam.sweep.n_red), "mean_iters_per_system": float(rf.iters.mean()),
-                 "max_dS_vs_sequential": float(max(np.abs(Sf[m] - seq[m].S).max() for m in range(len(probs))))}
+        unions = {}
+        for segd in (True, False):
+            fam = FamilySweep(probs, [w.waveguide_bcs(q) for q in probs], segmented=segd)
+            fam.solve(freqs[:2], precision="mixed", **kf)
+            t0 = time.perf_counter()
+            Sf, bound, rf = fam.solve(freqs, precision="mixed", **kf)
+            tf = time.perf_counter() - t0
+            ok_f = int(np.sum(np.all(bound <= 1e-8, axis=1))) * len(probs)
+            unions["one_system" if segd else "one_system_global_scalars"] = {
+                "seconds": round(tf, 3), "converged_points": ok_f, "freq_points_per_s": round(ok_f / tf, 3),
+                "reduced_dofs": int(fam.sweep.n_red), "max_iters_per_system": int(rf.iters.max()),
+                "mean_iters_per_system": float(rf.iters.mean()),
+                "max_dS_vs_sequential": float(max(np.abs(Sf[m] - seq[m].S).max() for m in range(len(probs))))}
+            fam.ctx.close()
         out.update(workload="16 jittered WR-90 sections (Kuhn 11x5xnz), DOFs U[20k,120k] seed 0",
                    dofs=[int(q.basis.n_field) for q in probs], points_per_mesh=len(freqs),
                    sequential={"seconds": round(ts, 3), "converged": ok_s, "freq_points_per_s": round(ok_s / ts, 3)},
                    concurrent={"seconds": round(tc, 3), "converged": ok_c, "freq_points_per_s": round(ok_c / tc, 3)},
                    max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))),
-                   one_system=union,
+                   **unions,
                    mean_iters_per_system_single=float(np.mean([r.iters.mean() for r in seq])))
     elif name == "farfield":
         out.update(farfield_bench())

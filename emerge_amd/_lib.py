@@ -63,6 +63,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_solver_set_escalation": (_c_int, [_p, _c_int]),
     "emg_solver_set_safe32": (_c_int, [_p, _c_int]),
     "emg_solver_set_spmv_lanes": (_c_int, [_p, _c_int]),
+    "emg_solver_set_segments": (_c_int, [_p, ctypes.c_int32, _p]),
     "emg_topology_build": (_c_int, [_p, _i64, _p, _i64, _p, _p, _p]),
     "emg_topology_get": (_c_int, [_p, _p, _p, _p, _p, _p, _p, _p]),
     "emg_port_points_set": (_c_int, [_p, ctypes.c_int32, _p, _p, _p, _p, _p]),

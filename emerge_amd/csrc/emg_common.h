@@ -162,6 +162,13 @@ struct emg_ctx {
     // Krylov method of the solves (emg_solver_config)
     int krylov_method = 0;        // 0 = GMRES(m), 1 = IDR(s), 2 = COCG on the symmetric part
     int spmv_gr = 0;              // COCG SpMV: entries of a row in flight (0 = default for the batch width)
+    // row segments of the reduced system with Krylov scalars of their own
+    // (emg_solver_set_segments; independent meshes side by side)
+    int seg_nseg = 0;
+    int64_t seg_n = 0;                  // reduced rows the segments were set for
+    std::vector<int64_t> seg_row_h;     // nseg + 1 first rows
+    emg::DBuf<int64_t> seg_row;
+    emg::DBuf<int32_t> seg_rseg;        // segment of each row
     int idr_s = 4;
     double inner_rel32 = 1e-5;    // FP32 inner reduction per refinement step
 

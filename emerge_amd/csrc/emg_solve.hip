@@ -911,14 +911,54 @@ constexpr int kCgMaxBlocks = 1024;  // grid cap of the SpMV (partial sums per ke
 constexpr int kCgUpdBlocks = 256;   // grid cap of the update pass
 
 struct CgState {
-    double2* rho;    // [F] r^T z
-    double2* beta;   // [F]
-    double2* alpha;  // [F]
-    double* rr;      // [F] ||r||^2 of the (scaled) inner residual
-    double* tgt2;    // [F] inner stop: rr <= tgt2
-    int* done;       // [F] inner solve finished (converged or broke down)
+    double2* rho;    // [S F] r^T z
+    double2* beta;   // [S F]
+    double2* alpha;  // [S F]
+    double* rr;      // [S F] ||r||^2 of the (scaled) inner residual
+    double* tgt2;    // [S F] inner stop: rr <= tgt2
+    int* done;       // [S F] inner solve finished (converged or broke down)
     double2* part;   // [kCgMaxBlocks][2F] partial sums
+    // row segments with Krylov scalars of their own (S = nseg; 1: one set
+    // per system).  Segment s holds rows [srow[s], srow[s+1]); the streaming
+    // kernels give it workgroups [sblk[s], sblk[s+1]), so that every
+    // workgroup's partial sums belong to one segment.
+    int nseg = 1;
+    const int* sblk = nullptr;      // [S + 1]
+    const int64_t* srow = nullptr;  // [S + 1]
+    const int* rseg = nullptr;      // [n] segment of each row (nullptr when S = 1)
 };
+
+// Scalar slot (segment, system) of element k = row F + f.
+__device__ __forceinline__ int seg_slot(const int* rseg, int64_t k, int F) {
+    return (rseg ? rseg[k / F] * F : 0) + int(k % F);
+}
+
+// The [row][f] elements a workgroup of a streaming COCG kernel walks: with
+// one segment the whole vector, grid-strided; otherwise the elements of its
+// own segment, strided over that segment's workgroups.  k0 and the stride
+// are multiples of kCgThreads, so thread t always holds system t % F.
+struct SegRange {
+    int64_t k0, k1, stride;
+    int s;
+};
+__device__ __forceinline__ SegRange seg_range(const CgState& st, int64_t nq, int F) {
+    SegRange r;
+    if (st.nseg <= 1) {
+        r.k0 = int64_t(blockIdx.x) * kCgThreads;
+        r.k1 = nq;
+        r.stride = int64_t(gridDim.x) * kCgThreads;
+        r.s = 0;
+        return r;
+    }
+    int s = 0;
+    while (s + 1 < st.nseg && int(blockIdx.x) >= st.sblk[s + 1]) ++s;
+    const int lb = int(blockIdx.x) - st.sblk[s], nb = st.sblk[s + 1] - st.sblk[s];
+    r.k0 = st.srow[s] * F + int64_t(lb) * kCgThreads;
+    r.k1 = st.srow[s + 1] * F;
+    r.stride = int64_t(nb) * kCgThreads;
+    r.s = s;
+    return r;
+}
 
 // slot of (col, row) for every slot of a structurally symmetric CSR pattern
 __global__ void k_tmap(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
@@ -1221,14 +1261,18 @@ void launch_cg_spmv(int gr, unsigned grid, hipStream_t s, const Op<R>& op, const
     k_cg_spmv<R, F, A, 0><<<grid, kCgThreads, 0, s>>>(op, k2, gam, p, q, cs);
 }
 
-// alpha = rho / p^T q (one workgroup; partials of k_cg_spmv)
+// alpha = rho / p^T q (one workgroup per segment; partials of k_cg_spmv,
+// or of k_cg_pq with several segments)
 template <int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_alpha(int nblk, CgState st) {
     __shared__ double2 red[kCgThreads];
-    cg_sum_partials(st.part, nblk, F, red);
+    const int sg = blockIdx.x;
+    const int b0 = st.nseg > 1 ? st.sblk[sg] : 0;
+    const int nb = st.nseg > 1 ? st.sblk[sg + 1] - b0 : nblk;
+    cg_sum_partials(st.part + int64_t(b0) * F, nb, F, red);
     if (threadIdx.x < F) {
-        const int ff = threadIdx.x;
-        const double2 pq = red[ff];
+        const int ff = sg * F + threadIdx.x;
+        const double2 pq = red[threadIdx.x];
         double2 al = make_double2(0.0, 0.0);
         if (!st.done[ff]) {
             const double m2 = pq.x * pq.x + pq.y * pq.y;
@@ -1239,12 +1283,57 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_alpha(int nblk, CgState st) {
     }
 }
 
+// Workgroup sums of the [row][f] products a_k b_k (conj(a_k) b_k with
+// CONJ) over the elements seg_range gives it: part[blockIdx][F].
+template <typename R, int F, bool CONJ>
+__global__ __launch_bounds__(kCgThreads) void k_seg_dot(int64_t nq, const cplx<R>* __restrict__ a,
+                                                        const cplx<R>* __restrict__ b, CgState st,
+                                                        double2* __restrict__ part) {
+    __shared__ double2 red[kCgThreads];
+    const int tid = threadIdx.x;
+    const SegRange sr = seg_range(st, nq, F);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t k = sr.k0 + tid; k < sr.k1; k += sr.stride) {
+        const double2 u = to_d(a[k]), v = to_d(b[k]);
+        if constexpr (CONJ) {
+            acc.x += u.x * v.x + u.y * v.y;
+            acc.y += u.x * v.y - u.y * v.x;
+        } else {
+            acc.x += u.x * v.x - u.y * v.y;
+            acc.y += u.x * v.y + u.y * v.x;
+        }
+    }
+    const int parts = kCgThreads / F, pi = tid / F;
+    red[tid] = acc;
+    __syncthreads();
+    for (int h = parts / 2; h >= 1; h >>= 1) {
+        if (pi < h) {
+            red[tid].x += red[tid + h * F].x;
+            red[tid].y += red[tid + h * F].y;
+        }
+        __syncthreads();
+    }
+    if (tid < F) part[int64_t(blockIdx.x) * F + tid] = red[tid];
+}
+
+// out[s F + f] = sum of the k_seg_dot partials of segment s (one workgroup
+// per segment)
+template <int F>
+__global__ __launch_bounds__(kCgThreads) void k_seg_dot_final(const double2* __restrict__ part, CgState st,
+                                                              double2* __restrict__ out) {
+    __shared__ double2 red[kCgThreads];
+    const int sg = blockIdx.x;
+    const int b0 = st.sblk[sg];
+    cg_sum_partials(part + int64_t(b0) * F, st.sblk[sg + 1] - b0, F, red);
+    if (threadIdx.x < F) out[sg * F + threadIdx.x] = red[threadIdx.x];
+}
+
 // p = z + beta p (beta = 0 on the first iteration of an inner solve)
 template <typename R, int F>
 __global__ void k_cg_pupd(int64_t nq, const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p, CgState st) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nq) return;
-    const cplx<R> b = from_d<R>(st.beta[k % F]);
+    const cplx<R> b = from_d<R>(st.beta[seg_slot(st.rseg, k, F)]);
     p[k] = cfma(b, p[k], z[k]);
 }
 
@@ -1259,10 +1348,11 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(int64_t nq, cplx<R>* _
     __shared__ double2 red[kCgThreads];
     const int tid = threadIdx.x;
     const int f = tid % F;  // blockDim and the grid stride are multiples of F
-    const cplx<R> al = init ? cmk<R>(R(0), R(0)) : from_d<R>(st.alpha[f]);
+    const SegRange sr = seg_range(st, nq, F);
+    const cplx<R> al = init ? cmk<R>(R(0), R(0)) : from_d<R>(st.alpha[sr.s * F + f]);
     double2 rz = make_double2(0.0, 0.0);
     double rr = 0.0;
-    for (int64_t k = int64_t(blockIdx.x) * kCgThreads + tid; k < nq; k += int64_t(gridDim.x) * kCgThreads) {
+    for (int64_t k = sr.k0 + tid; k < sr.k1; k += sr.stride) {
         cplx<R> rv = r[k];
         if (init) {
             dx[k] = cmk<R>(R(0), R(0));
@@ -1313,15 +1403,19 @@ template <int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_beta(int nblk, CgState st, int init) {
     __shared__ double2 red[kCgThreads];
     const int tid = threadIdx.x;
-    cg_sum_partials(st.part, nblk, 2 * F, red);
+    const int sg = blockIdx.x;  // one workgroup per segment
+    const int b0 = st.nseg > 1 ? st.sblk[sg] : 0;
+    const int nb = st.nseg > 1 ? st.sblk[sg + 1] - b0 : nblk;
+    cg_sum_partials(st.part + int64_t(b0) * 2 * F, nb, 2 * F, red);
     if (tid < F) {
+        const int sf = sg * F + tid;
         const double2 rho1 = red[tid];
         const double rrv = red[F + tid].x;
-        st.rr[tid] = rrv;
-        int dn = init ? 0 : st.done[tid];
+        st.rr[sf] = rrv;
+        int dn = init ? 0 : st.done[sf];
         double2 be = make_double2(0.0, 0.0);
-        if (!(rrv <= st.tgt2[tid])) {
-            const double2 rho0 = st.rho[tid];
+        if (!(rrv <= st.tgt2[sf])) {
+            const double2 rho0 = st.rho[sf];
             const double m2 = rho0.x * rho0.x + rho0.y * rho0.y;
             if (!init && !dn) {
                 if (m2 > 0 && isfinite(m2)) be = cdivd(rho1, rho0);
@@ -1331,19 +1425,19 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_beta(int nblk, CgState st, in
             dn = 1;
         }
         if (!isfinite(rrv)) dn = 1;
-        st.done[tid] = dn;
-        st.rho[tid] = rho1;
-        st.beta[tid] = be;
+        st.done[sf] = dn;
+        st.rho[sf] = rho1;
+        st.beta[sf] = be;
     }
 }
 
 // r = cast(r64 / ||r64||) per system
 template <typename R, int F>
 __global__ void k_cast_scaled(int64_t nq, const double2* __restrict__ src, const double2* __restrict__ nrm,
-                              cplx<R>* __restrict__ dst) {
+                              cplx<R>* __restrict__ dst, const int* __restrict__ rseg) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nq) return;
-    const double n2 = nrm[k % F].x;
+    const double n2 = nrm[seg_slot(rseg, k, F)].x;
     const double sc = n2 > 0 ? 1.0 / sqrt(n2) : 0.0;
     dst[k] = from_d<R>(make_double2(src[k].x * sc, src[k].y * sc));
 }
@@ -1351,10 +1445,10 @@ __global__ void k_cast_scaled(int64_t nq, const double2* __restrict__ src, const
 // x64 += ||r64|| dx per system
 template <typename R, int F>
 __global__ void k_cg_accum(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm,
-                           double2* __restrict__ x) {
+                           double2* __restrict__ x, const int* __restrict__ rseg) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nq) return;
-    const double sc = sqrt(fmax(nrm[k % F].x, 0.0));
+    const double sc = sqrt(fmax(nrm[seg_slot(rseg, k, F)].x, 0.0));
     const double2 d = to_d(dx[k]);
     x[k].x += sc * d.x;
     x[k].y += sc * d.y;
@@ -1481,7 +1575,8 @@ __global__ __launch_bounds__(kCgThreads) void k_aux_z(int64_t nq, Op<R> op, cons
     __shared__ double2 red[kCgThreads];
     const int tid = threadIdx.x;
     double2 rz = make_double2(0.0, 0.0);
-    for (int64_t k = int64_t(blockIdx.x) * kCgThreads + tid; k < nq; k += int64_t(gridDim.x) * kCgThreads) {
+    const SegRange sr = seg_range(st, nq, F);
+    for (int64_t k = sr.k0 + tid; k < sr.k1; k += sr.stride) {
         const int64_t row = k / F;
         const int f = int(k % F);
         const cplx<R> rv = r[k];
@@ -1522,10 +1617,10 @@ template <typename R, int F>
 __global__ void k_mr_apply(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm_prev,
                            const double2* __restrict__ om, double2* __restrict__ x,
                            const double2* __restrict__ r_old, const double2* __restrict__ d,
-                           double2* __restrict__ r) {
+                           double2* __restrict__ r, const int* __restrict__ rseg) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nq) return;
-    const int f = int(k % F);
+    const int f = seg_slot(rseg, k, F);
     const double2 w = om[f];
     const double sc = sqrt(fmax(nrm_prev[f].x, 0.0));
     const double2 s = to_d(dx[k]);
@@ -1539,11 +1634,14 @@ __global__ void k_mr_apply(int64_t nq, const cplx<R>* __restrict__ dx, const dou
 // undo the last refinement step of the flagged systems: x64 -= w ||r64_prev|| dx
 template <typename R, int F>
 __global__ void k_cg_rollback(int64_t nq, const cplx<R>* __restrict__ dx, const double2* __restrict__ nrm,
-                              const double2* __restrict__ om, const int* __restrict__ flag, double2* __restrict__ x) {
+                              const double2* __restrict__ om, const int* __restrict__ flag, double2* __restrict__ x,
+                              const int* __restrict__ rseg) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= nq || !flag[k % F]) return;
-    const double sc = sqrt(fmax(nrm[k % F].x, 0.0));
-    const double2 w = om[k % F];
+    if (k >= nq) return;
+    const int sf = seg_slot(rseg, k, F);
+    if (!flag[sf]) return;
+    const double sc = sqrt(fmax(nrm[sf].x, 0.0));
+    const double2 w = om[sf];
     const double2 d = to_d(dx[k]);
     x[k].x -= sc * (w.x * d.x - w.y * d.y);
     x[k].y -= sc * (w.x * d.y + w.y * d.x);
@@ -1554,6 +1652,11 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
              const double2* b64, double2* x64, const SolveParams& P, BatchStats& st) {
     hipStream_t s = c->stream;
     const int64_t n = op.n, nq = n * F;
+    // row segments (emg_solver_set_segments) of this system: S sets of
+    // Krylov scalars per batch, NS = S F inner systems
+    const bool segd = c->seg_nseg > 1 && c->seg_n == n;
+    const int S = segd ? c->seg_nseg : 1;
+    const int NS = S * F;
     DBuf<cplx<R>> pv, q, r, z, dx, dinv;
     DBuf<double2> r64, t64, nrm, bn, partial, rho, beta, alpha;
     DBuf<double> rr, tgt2, k264;
@@ -1578,10 +1681,10 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     }
     EMG_HIP(c, r64.alloc(nq));
     EMG_HIP(c, t64.alloc(nq));
-    for (DBuf<double2>* b : {&nrm, &bn, &rho, &beta, &alpha}) EMG_HIP(c, b->alloc(F));
-    EMG_HIP(c, rr.alloc(F));
-    EMG_HIP(c, tgt2.alloc(F));
-    EMG_HIP(c, done.alloc(F));
+    for (DBuf<double2>* b : {&nrm, &bn, &rho, &beta, &alpha}) EMG_HIP(c, b->alloc(NS));
+    EMG_HIP(c, rr.alloc(NS));
+    EMG_HIP(c, tgt2.alloc(NS));
+    EMG_HIP(c, done.alloc(NS));
     const int64_t nchunk = (nq + 511) / 512;
     EMG_HIP(c, partial.alloc(std::max<int64_t>(nchunk * F, int64_t(kCgMaxBlocks) * 2 * F)));
     EMG_HIP(c, k2d.alloc(F));
@@ -1612,10 +1715,34 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     cs.done = done.p;
     cs.part = partial.p;
     const unsigned gq = grid_for(nq, 256);
+    // streaming grid: gb workgroups over the whole vector, or per segment a
+    // share of them proportional to its rows (at least one)
+    const unsigned gb0 = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
+    DBuf<int> sblk;
+    unsigned gs = gb0;
+    const int* rseg = nullptr;
+    if (segd) {
+        std::vector<int> sb(S + 1, 0);
+        for (int g = 0; g < S; ++g) {
+            const int64_t rows = c->seg_row_h[g + 1] - c->seg_row_h[g];
+            const int64_t cap = std::max<int64_t>(1, (rows * F + kCgThreads - 1) / kCgThreads);
+            const int64_t want = std::llround(double(gb0) * double(rows) / double(n));
+            sb[g + 1] = sb[g] + int(std::max<int64_t>(1, std::min(cap, want)));
+        }
+        if (sb[S] > kCgMaxBlocks) return fail(c, EMG_ERR_ARG, "COCG: %d segments need too many workgroups", S);
+        EMG_HIP(c, sblk.alloc(S + 1));
+        EMG_HIP(c, hipMemcpyAsync(sblk.p, sb.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice, s));
+        gs = unsigned(sb[S]);
+        cs.nseg = S;
+        cs.sblk = sblk.p;
+        cs.srow = c->seg_row.p;
+        cs.rseg = c->seg_rseg.p;
+        rseg = c->seg_rseg.p;
+    }
     const unsigned grow = grid_for(n, 4);
     const unsigned gdots = grid_for(nchunk * 64, 256);
     const unsigned ga = unsigned(8 * std::min<int64_t>(kCgMaxBlocks / 8, std::max<int64_t>(1, (n + 127) / 128)));
-    const unsigned gb = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
+    const unsigned gb = gs;
     k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
     const unsigned gaux = aux ? grid_for(op.aux_n * F, 256) : 1;
     // k_aux_gt: 64 / LPR nodal rows per wavefront, 4 wavefronts per workgroup
@@ -1643,18 +1770,25 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             k_aux_z<R, F><<<gb, kCgThreads, 0, s>>>(nq, op, dinv.p, r.p, tg.p, z.p, cs);
         }
     };
-    auto dots64 = [&](const double2* a, double2* out) {
-        k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, a, partial.p);
-        k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
+    // a^H b per inner system (per segment and system with segments)
+    auto dot64 = [&](const double2* a, const double2* b, double2* out) {
+        if (segd) {
+            k_seg_dot<double, F, true><<<gs, kCgThreads, 0, s>>>(nq, a, b, cs, partial.p);
+            k_seg_dot_final<F><<<S, kCgThreads, 0, s>>>(partial.p, cs, out);
+        } else {
+            k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, b, partial.p);
+            k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
+        }
     };
-    std::vector<double2> hb(F), hr(F);
+    auto dots64 = [&](const double2* a, double2* out) { dot64(a, a, out); };
+    std::vector<double2> hb(NS), hr(NS);
     dots64(b64, bn.p);
-    EMG_HIP(c, hipMemcpyAsync(hb.data(), bn.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+    EMG_HIP(c, hipMemcpyAsync(hb.data(), bn.p, sizeof(double2) * NS, hipMemcpyDeviceToHost, s));
     EMG_HIP(c, hipStreamSynchronize(s));
-    std::vector<double> bnorm(F);
-    for (int f = 0; f < F; ++f) bnorm[f] = std::sqrt(std::max(hb[f].x, 0.0));
-    st.relres.assign(F, 1.0);
-    st.iters.assign(F, 0);
+    std::vector<double> bnorm(NS);
+    for (int f = 0; f < NS; ++f) bnorm[f] = std::sqrt(std::max(hb[f].x, 0.0));
+    st.relres.assign(NS, 1.0);
+    st.iters.assign(NS, 0);
     // stop flags land in pinned host memory; the graph of `blk` iterations
     // is built on first use and freed on every exit path
     struct Res {
@@ -1667,8 +1801,8 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             if (hr) (void)hipHostFree(hr);
         }
     } res;
-    EMG_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&res.h), sizeof(int) * F, hipHostMallocDefault));
-    EMG_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&res.hr), sizeof(double) * F, hipHostMallocDefault));
+    EMG_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&res.h), sizeof(int) * NS, hipHostMallocDefault));
+    EMG_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&res.hr), sizeof(double) * NS, hipHostMallocDefault));
     int* hdone = res.h;
     double* hrr = res.hr;
     // an inner solve whose residual has not halved for this many iterations
@@ -1681,29 +1815,25 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     const bool use_graph = !(ge && ge[0] == '0');
     hipGraphExec_t& graph = res.g;
     const int blk = std::max(2, (P.check_every + 1) & ~1);
-    std::vector<double> ht(F);
+    std::vector<double> ht(NS);
     // refinement guard: a system whose FP64 residual grows past 4x its best,
     // or has not improved for 3 cycles, is given up (reported unconverged)
     // (a step that made the residual worse is undone first)
-    std::vector<double> best_out(F, HUGE_VAL), prev_rel(F, 1.0);
-    std::vector<int> cyc_best(F, 0), gave_up(F, 0), rb(F, 0);
+    std::vector<double> best_out(NS, HUGE_VAL), prev_rel(NS, 1.0);
+    std::vector<int> cyc_best(NS, 0), gave_up(NS, 0), rb(NS, 0);
     DBuf<double2> nrm_prev, r_old, dn, dd, omd;
     DBuf<int> rbd;
-    EMG_HIP(c, nrm_prev.alloc(F));
-    EMG_HIP(c, rbd.alloc(F));
+    EMG_HIP(c, nrm_prev.alloc(NS));
+    EMG_HIP(c, rbd.alloc(NS));
     EMG_HIP(c, r_old.alloc(nq));
-    EMG_HIP(c, dn.alloc(F));
-    EMG_HIP(c, dd.alloc(F));
-    EMG_HIP(c, omd.alloc(F));
-    auto dot64 = [&](const double2* a, const double2* b, double2* out) {  // a^H b per system
-        k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, b, partial.p);
-        k_dots_final<<<F, 64, 0, s>>>(nchunk, 1, F, partial.p, out, 0);
-    };
-    std::vector<double2> hdn(F), hdd(F), hom(F, make_double2(1.0, 0.0));
-    std::vector<int> stepped(F, 0);  // the system took an inner correction last cycle
-    EMG_HIP(c, hipMemcpyAsync(omd.p, hom.data(), sizeof(double2) * F, hipMemcpyHostToDevice, s));
+    EMG_HIP(c, dn.alloc(NS));
+    EMG_HIP(c, dd.alloc(NS));
+    EMG_HIP(c, omd.alloc(NS));
+    std::vector<double2> hdn(NS), hdd(NS), hom(NS, make_double2(1.0, 0.0));
+    std::vector<int> stepped(NS, 0);  // the system took an inner correction last cycle
+    EMG_HIP(c, hipMemcpyAsync(omd.p, hom.data(), sizeof(double2) * NS, hipMemcpyHostToDevice, s));
     for (int cyc = 0; cyc < P.max_cycles; ++cyc) {
-        if (cyc > 0) EMG_HIP(c, hipMemcpyAsync(nrm_prev.p, nrm.p, sizeof(double2) * F, hipMemcpyDeviceToDevice, s));
+        if (cyc > 0) EMG_HIP(c, hipMemcpyAsync(nrm_prev.p, nrm.p, sizeof(double2) * NS, hipMemcpyDeviceToDevice, s));
         k_spmv<double, F><<<grow, 256, 0, s>>>(op64, k264.p, gam64.p, x64, nullptr, t64.p);
         k_sub<<<gq, 256, 0, s>>>(nq, b64, t64.p, r64.p);
         if (cyc > 0) {
@@ -1711,26 +1841,26 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             k_sub<<<gq, 256, 0, s>>>(nq, r_old.p, r64.p, t64.p);
             dot64(t64.p, r_old.p, dn.p);
             dot64(t64.p, t64.p, dd.p);
-            EMG_HIP(c, hipMemcpyAsync(hdn.data(), dn.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
-            EMG_HIP(c, hipMemcpyAsync(hdd.data(), dd.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipMemcpyAsync(hdn.data(), dn.p, sizeof(double2) * NS, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipMemcpyAsync(hdd.data(), dd.p, sizeof(double2) * NS, hipMemcpyDeviceToHost, s));
             EMG_HIP(c, hipStreamSynchronize(s));
-            for (int f = 0; f < F; ++f) {
+            for (int f = 0; f < NS; ++f) {
                 const double den = hdd[f].x;
                 hom[f] = (stepped[f] && den > 0 && std::isfinite(den) && std::isfinite(hdn[f].x) &&
                           std::isfinite(hdn[f].y))
                              ? make_double2(hdn[f].x / den, hdn[f].y / den)
                              : make_double2(1.0, 0.0);
             }
-            EMG_HIP(c, hipMemcpyAsync(omd.p, hom.data(), sizeof(double2) * F, hipMemcpyHostToDevice, s));
-            k_mr_apply<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, omd.p, x64, r_old.p, t64.p, r64.p);
+            EMG_HIP(c, hipMemcpyAsync(omd.p, hom.data(), sizeof(double2) * NS, hipMemcpyHostToDevice, s));
+            k_mr_apply<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, omd.p, x64, r_old.p, t64.p, r64.p, rseg);
         }
         dots64(r64.p, nrm.p);
         EMG_HIP(c, hipMemcpyAsync(r_old.p, r64.p, sizeof(double2) * nq, hipMemcpyDeviceToDevice, s));
-        EMG_HIP(c, hipMemcpyAsync(hr.data(), nrm.p, sizeof(double2) * F, hipMemcpyDeviceToHost, s));
+        EMG_HIP(c, hipMemcpyAsync(hr.data(), nrm.p, sizeof(double2) * NS, hipMemcpyDeviceToHost, s));
         EMG_HIP(c, hipStreamSynchronize(s));
         EMG_HIP(c, hipGetLastError());
         bool all = true;
-        for (int f = 0; f < F; ++f) {
+        for (int f = 0; f < NS; ++f) {
             const double rn = std::sqrt(std::max(hr[f].x, 0.0));
             st.relres[f] = bnorm[f] > 0 ? rn / bnorm[f] : 0.0;
             rb[f] = 0;
@@ -1756,17 +1886,17 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         }
         st.cycles = cyc;
         if (std::find(rb.begin(), rb.end(), 1) != rb.end()) {
-            EMG_HIP(c, hipMemcpyAsync(rbd.p, rb.data(), sizeof(int) * F, hipMemcpyHostToDevice, s));
-            k_cg_rollback<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, omd.p, rbd.p, x64);
+            EMG_HIP(c, hipMemcpyAsync(rbd.p, rb.data(), sizeof(int) * NS, hipMemcpyHostToDevice, s));
+            k_cg_rollback<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm_prev.p, omd.p, rbd.p, x64, rseg);
             EMG_HIP(c, hipGetLastError());
         }
         if (all) break;
-        for (int f = 0; f < F; ++f) stepped[f] = ht[f] < 1.0;
-        EMG_HIP(c, hipMemcpyAsync(tgt2.p, ht.data(), sizeof(double) * F, hipMemcpyHostToDevice, s));
-        k_cast_scaled<R, F><<<gq, 256, 0, s>>>(nq, r64.p, nrm.p, r.p);
+        for (int f = 0; f < NS; ++f) stepped[f] = ht[f] < 1.0;
+        EMG_HIP(c, hipMemcpyAsync(tgt2.p, ht.data(), sizeof(double) * NS, hipMemcpyHostToDevice, s));
+        k_cast_scaled<R, F><<<gq, 256, 0, s>>>(nq, r64.p, nrm.p, r.p, rseg);
         k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 1, int(aux));
         precondition();
-        k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 1);
+        k_cg_beta<F><<<S, kCgThreads, 0, s>>>(int(gb), cs, 1);
         EMG_HIP(c, hipGetLastError());
         auto iterations = [&]() {
             for (int i = 0; i < blk; ++i) {
@@ -1775,10 +1905,13 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
                     launch_cg_spmv<R, F, double>(c->spmv_gr, ga, s, op, k2d.p, gam.p, pv.p, q.p, cs);
                 else
                     launch_cg_spmv<R, F, R>(c->spmv_gr, ga, s, op, k2d.p, gam.p, pv.p, q.p, cs);
-                k_cg_alpha<F><<<1, kCgThreads, 0, s>>>(int(ga), cs);
+                // with segments p^T q is summed per segment by its own pass
+                // (the SpMV's workgroups span segments)
+                if (segd) k_seg_dot<R, F, false><<<gs, kCgThreads, 0, s>>>(nq, pv.p, q.p, cs, partial.p);
+                k_cg_alpha<F><<<S, kCgThreads, 0, s>>>(int(ga), cs);
                 k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0, int(aux));
                 precondition();
-                k_cg_beta<F><<<1, kCgThreads, 0, s>>>(int(gb), cs, 0);
+                k_cg_beta<F><<<S, kCgThreads, 0, s>>>(int(gb), cs, 0);
             }
         };
         if (!graph && use_graph) {
@@ -1793,11 +1926,11 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             if (ei != hipSuccess) return fail(c, EMG_ERR_HIP, "COCG graph instantiate failed: %s", hipGetErrorString(ei));
         }
         int it = 0;
-        std::vector<int> active(F), it_best(F, 0);
-        std::vector<double> best(F, 1.0);
+        std::vector<int> active(NS), it_best(NS, 0);
+        std::vector<double> best(NS, 1.0);
         // systems with an inner target this cycle (finished and given-up
         // ones got a target they already meet)
-        for (int f = 0; f < F; ++f) active[f] = ht[f] < 1.0;
+        for (int f = 0; f < NS; ++f) active[f] = ht[f] < 1.0;
         // a block runs until its slowest system stops: once the first system
         // has converged at iteration k1, complex64 systems still running past
         // max(4 k1, k1 + 400) are stopped and restarted by the refinement
@@ -1810,17 +1943,17 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
             else iterations();
             EMG_HIP(c, hipGetLastError());
             it += blk;
-            for (int f = 0; f < F; ++f) st.iters[f] += active[f] ? blk : 0;
-            EMG_HIP(c, hipMemcpyAsync(hdone, done.p, sizeof(int) * F, hipMemcpyDeviceToHost, s));
-            EMG_HIP(c, hipMemcpyAsync(hrr, rr.p, sizeof(double) * F, hipMemcpyDeviceToHost, s));
+            for (int f = 0; f < NS; ++f) st.iters[f] += active[f] ? blk : 0;
+            EMG_HIP(c, hipMemcpyAsync(hdone, done.p, sizeof(int) * NS, hipMemcpyDeviceToHost, s));
+            EMG_HIP(c, hipMemcpyAsync(hrr, rr.p, sizeof(double) * NS, hipMemcpyDeviceToHost, s));
             EMG_HIP(c, hipStreamSynchronize(s));
             bool fin = true, stop = false;
             if (first_done < 0)
-                for (int f = 0; f < F; ++f)
+                for (int f = 0; f < NS; ++f)
                     if (active[f] && hdone[f]) first_done = it;
             const bool cut = sizeof(R) < sizeof(double) && first_done >= 0 &&
                              it > std::max(4 * first_done, first_done + 400);
-            for (int f = 0; f < F; ++f) {
+            for (int f = 0; f < NS; ++f) {
                 if (cut && !hdone[f] && active[f]) {
                     hdone[f] = 1;
                     stop = true;
@@ -1838,16 +1971,27 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
                 else fin = false;
             }
             if (fin) break;
-            if (stop) EMG_HIP(c, hipMemcpyAsync(done.p, hdone, sizeof(int) * F, hipMemcpyHostToDevice, s));
+            if (stop) EMG_HIP(c, hipMemcpyAsync(done.p, hdone, sizeof(int) * NS, hipMemcpyHostToDevice, s));
         }
         if (getenv("EMG_COCG_TRACE")) {
             std::fprintf(stderr, "[cocg] F=%d cycle %d: %d inner iterations, relres", F, cyc, it);
-            for (int f = 0; f < F; ++f) std::fprintf(stderr, " %.1e", st.relres[f]);
+            for (int f = 0; f < NS; ++f) std::fprintf(stderr, " %.1e", st.relres[f]);
             std::fprintf(stderr, "\n");
         }
-        k_cg_accum<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm.p, x64);
+        k_cg_accum<R, F><<<gq, 256, 0, s>>>(nq, dx.p, nrm.p, x64, rseg);
         EMG_HIP(c, hipGetLastError());
         st.cycles = cyc + 1;
+    }
+    if (segd) {
+        // a system's figures are those of its slowest segment (each segment's
+        // residual relative to its own right-hand side)
+        for (int f = 0; f < F; ++f)
+            for (int g = 1; g < S; ++g) {
+                st.relres[f] = std::max(st.relres[f], st.relres[g * F + f]);
+                st.iters[f] = std::max(st.iters[f], st.iters[g * F + f]);
+            }
+        st.relres.resize(F);
+        st.iters.resize(F);
     }
     return EMG_OK;
 }
@@ -1916,6 +2060,45 @@ extern "C" int emg_solver_set_spmv_lanes(emg_ctx* c, int entries_in_flight) {
     if (g != 0 && g != 4 && g != 8 && g != 16 && g != 32)
         return fail(c, EMG_ERR_ARG, "emg_solver_set_spmv_lanes: %d not in {0, 4, 8, 16, 32}", g);
     c->spmv_gr = g;
+    return EMG_OK;
+}
+
+namespace {
+__global__ void k_row_segments(int nseg, const int64_t* __restrict__ row0, int32_t* __restrict__ rseg) {
+    const int g = blockIdx.y;
+    if (g >= nseg) return;
+    for (int64_t r = row0[g] + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < row0[g + 1];
+         r += int64_t(gridDim.x) * blockDim.x)
+        rseg[r] = g;
+}
+}  // namespace
+
+extern "C" int emg_solver_set_segments(emg_ctx* c, int32_t nseg, const int64_t* row_start) {
+    EMG_CHECK_CTX(c);
+    if (nseg <= 1) {
+        c->seg_nseg = 0;
+        c->seg_n = 0;
+        c->seg_row_h.clear();
+        return EMG_OK;
+    }
+    if (!c->have_system) return fail(c, EMG_ERR_STATE, "emg_solver_set_segments: no reduced system");
+    if (nseg > 256 || !row_start) return fail(c, EMG_ERR_ARG, "emg_solver_set_segments: 2..256 segments");
+    const int64_t n = c->n_red;
+    if (row_start[0] != 0 || row_start[nseg] != n)
+        return fail(c, EMG_ERR_ARG, "emg_solver_set_segments: segments must cover rows 0..%lld", (long long)n);
+    for (int g = 0; g < nseg; ++g)
+        if (row_start[g + 1] <= row_start[g])
+            return fail(c, EMG_ERR_ARG, "emg_solver_set_segments: empty or decreasing segment %d", g);
+    hipStream_t s = c->stream;
+    c->seg_row_h.assign(row_start, row_start + nseg + 1);
+    EMG_HIP(c, c->seg_row.alloc(nseg + 1));
+    EMG_HIP(c, c->seg_rseg.alloc(n));
+    EMG_HIP(c, hipMemcpyAsync(c->seg_row.p, c->seg_row_h.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s));
+    k_row_segments<<<dim3(64, nseg), 256, 0, s>>>(nseg, c->seg_row.p, c->seg_rseg.p);
+    EMG_HIP(c, hipGetLastError());
+    EMG_HIP(c, hipStreamSynchronize(s));
+    c->seg_nseg = nseg;
+    c->seg_n = n;
     return EMG_OK;
 }
 
@@ -2079,6 +2262,8 @@ extern "C" int emg_sweep_solve(emg_ctx* c, int64_t nbatch, const double* k2, con
     if (precision < 0 || precision > 2) return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision must be 0, 1 or 2");
     if (precision == 2 && c->krylov_method != 2)
         return fail(c, EMG_ERR_ARG, "emg_sweep_solve: precision 2 (complex64 mass values) needs COCG");
+    if (c->seg_nseg > 1 && c->seg_n == c->n_red && c->krylov_method != 2)
+        return fail(c, EMG_ERR_ARG, "emg_sweep_solve: row segments need COCG");
     fblock = std::max(1, std::min(fblock, c->krylov_method == 2 ? 64 : 32));
     hipStream_t s = c->stream;
     const int64_t n = c->n_red;

@@ -11,9 +11,13 @@ excited in the same system ("group" p), so a system's right-hand side is
 the sum of the meshes' own right-hand sides and its solution restricted to
 mesh m is mesh m's solution for its port p.
 
-The Krylov scalars of the batched solve are global over the union, so the
-union system is solved to tol * min_m ||b_m|| / ||b||: that bounds every
-mesh's own relative residual ||b_m - A_m x_m|| / ||b_m|| by tol.
+By default (``segmented=True``) the reduced rows are grouped by mesh and the
+COCG solve keeps one set of Krylov scalars per mesh (emg_solver_set_segments):
+every mesh follows the recurrence it would follow alone, with its own stop
+test on ||b_m - A_m x_m|| / ||b_m||, while each kernel launch still covers
+all meshes.  With ``segmented=False`` the scalars are global over the union,
+and the union system is solved to tol * min_m ||b_m|| / ||b||, which bounds
+every mesh's own relative residual by tol.
 """
 from __future__ import annotations
 
@@ -50,7 +54,8 @@ class FamilySweep:
     and its own boundary conditions, the same number of ports each) solved as
     one block-diagonal device system."""
 
-    def __init__(self, problems: list, boundary_conditions: list, device: int = 0, gap: float | None = None):
+    def __init__(self, problems: list, boundary_conditions: list, device: int = 0, gap: float | None = None,
+                 segmented: bool = True):
         if len(problems) != len(boundary_conditions) or not problems:
             raise ValueError("one list of boundary conditions per problem")
         nports = {sum(1 for b in bl if isinstance(b, bcs.PortBC)) for bl in boundary_conditions}
@@ -91,8 +96,21 @@ class FamilySweep:
             if not np.array_equal(self.mesh.tris[:, o:o + nt], p.mesh.tris + vo):
                 raise AssertionError("union triangle order differs from the meshes' order")
         self.basis = Nedelec2(self.mesh)
+        self.segmented = bool(segmented)
+        seg = None
+        if self.segmented:
+            # mesh of every field DOF, from the mesh of its edge or triangle
+            ub = self.basis
+            voff = np.asarray(self.vert_off, dtype=np.int64)
+            seg = np.full(ub.n_field, -1, dtype=np.int64)
+            e_mesh = np.searchsorted(voff, self.mesh.edges[0], side="right") - 1
+            t_mesh = np.searchsorted(voff, self.mesh.tris[0], side="right") - 1
+            seg[ub.edge_to_field] = e_mesh[None, :]
+            seg[ub.tri_to_field] = t_mesh[None, :]
+            if np.any(seg < 0):
+                raise AssertionError("a field DOF without a mesh")
         self.sweep = FrequencySweep(self.mesh, self.basis, np.concatenate(eps), np.concatenate(inu), all_bcs,
-                                    device=device)
+                                    device=device, dof_segments=seg)
         P = self.n_ports
         # group g = port g of every mesh; sweep.ports are sorted by port number
         self.groups = [[m * P + g for m in range(self.n_meshes)] for g in range(P)]
@@ -107,7 +125,8 @@ class FamilySweep:
         nrm = np.linalg.norm(F, axis=1)
         # min_m ||b_m|| / ||b|| per group (the factor uinc_factor(k0) is
         # common to a group, so the forcing shapes' norms give the ratio)
-        self.tol_scale = min(float(nrm[grp].min() / np.sqrt(np.sum(nrm[grp] ** 2))) for grp in self.groups)
+        self.tol_scale = 1.0 if self.segmented else min(
+            float(nrm[grp].min() / np.sqrt(np.sum(nrm[grp] ** 2))) for grp in self.groups)
 
     @property
     def ctx(self):

@@ -125,7 +125,7 @@ class FrequencySweep:
 
     def __init__(self, mesh, basis, eps_r, inv_mu_r, boundary_conditions: list,
                  device: int = 0, ctx: Context | None = None, dof_order: str = "spatial", mu_r=None,
-                 cond=None, peclim: float = PECLIM):
+                 cond=None, peclim: float = PECLIM, dof_segments=None):
         """Materials per tet: eps_r a complex scalar (T,) or tensor (3, 3, T);
         inv_mu_r complex scalars 1/mu_r (T,), or mu_r a tensor (3, 3, T) (the
         reference's ``ur``, inverted on the device) in its place.  Any of them
@@ -139,7 +139,11 @@ class FrequencySweep:
         perfect conductors (their DOFs are eliminated with the PEC faces, and
         their permittivity plays no part), the other conducting tets add
         -j cond / (w eps0) to eps_r at every frequency (the sweep is then
-        frequency-dependent; ASM/assembler.py:240-288)."""
+        frequency-dependent; ASM/assembler.py:240-288).  dof_segments: a
+        segment id per field DOF (N,) for a mesh made of independent parts
+        (no matrix entry couples two segments): the reduced rows are grouped
+        by segment and the COCG solves run one Krylov recurrence per segment
+        (``emg_solver_set_segments``), each stopped on its own residual."""
         self.conductor_tets = None
         if cond is not None:
             pec_t, lossy = conductor_split(cond, peclim)
@@ -164,6 +168,9 @@ class FrequencySweep:
         if (inv_mu_r is None) == (mu_r is None):
             raise ValueError("give exactly one of inv_mu_r (scalars) and mu_r (tensors)")
         self.dof_order = dof_order
+        self.dof_segments = None if dof_segments is None else np.asarray(dof_segments, dtype=np.int64)
+        if self.dof_segments is not None and self.dof_segments.shape != (basis.n_field,):
+            raise ValueError("dof_segments needs one segment id per field DOF")
         self.mesh = mesh
         self.basis = basis
         self.ctx = ctx if ctx is not None else Context(device)
@@ -334,12 +341,21 @@ class FrequencySweep:
         self.solve_ids = np.nonzero(keep_mask)[0]
         if self.dof_order == "spatial":
             self.solve_ids = self.spatial_order(self.solve_ids)
+        if self.dof_segments is not None:
+            # segment-major rows, the order above kept inside each segment
+            self.solve_ids = self.solve_ids[np.argsort(self.dof_segments[self.solve_ids], kind="stable")]
         red = np.full(N, -1, dtype=np.int64)
         red[self.solve_ids] = np.arange(self.solve_ids.shape[0])
         self.red = red
         ids32 = np.ascontiguousarray(self.solve_ids, dtype=np.int32)
         self.ctx.call("emg_system_reduce", ids32.shape[0], ptr(ids32))
         self.n_red = ids32.shape[0]
+        self.segment_rows = None
+        if self.dof_segments is not None:
+            sid = self.dof_segments[self.solve_ids]
+            starts = np.concatenate([[0], np.nonzero(np.diff(sid))[0] + 1, [sid.shape[0]]]).astype(np.int64)
+            self.segment_rows = np.ascontiguousarray(starts)
+            self.ctx.call("emg_solver_set_segments", int(starts.shape[0] - 1), ptr(self.segment_rows))
 
         # surface terms on the device, one per Robin BC plus one per
         # second-order ABC correction: (kind, bc); the solve scales term i by

@@ -251,6 +251,17 @@ int emg_solver_set_safe32(emg_ctx* ctx, int on);
  * systems per lane, more than lanes per wavefront) runs the default.  Used
  * by the host's autotuner; no reference counterpart. */
 int emg_solver_set_spmv_lanes(emg_ctx* ctx, int entries_in_flight);
+/* Row segments of the reduced system for the following COCG solves: rows
+ * [row_start[s], row_start[s+1]) form segment s (row_start[0] = 0,
+ * row_start[nseg] = the reduced size, increasing).  For a block-diagonal
+ * system of independent meshes placed side by side: each segment runs its
+ * own Krylov recurrence (its own scalars, stop test and refinement on its
+ * own right-hand side) inside the same kernel launches, so every mesh
+ * converges as it would alone; relres and iterations are reported per system
+ * as those of its slowest segment.  nseg <= 1 clears them.  At most 256
+ * segments; COCG only.  Replaces the reference's one solve per job
+ * (MW/microwave_3d.py:773-861, sequential, threaded or process-pool). */
+int emg_solver_set_segments(emg_ctx* ctx, int32_t nseg, const int64_t* row_start);
 
 /* Mesh topology on the device (replaces Mesh3D._pre_update,
  * E/mesh3d.py:386-498): unique edges / triangles of the tets (sorted vertex

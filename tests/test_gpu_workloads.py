@@ -86,22 +86,27 @@ def test_patch_antenna_reduced(precision):
     assert dS <= 1e-6
 
 
-def test_sixteen_uneven_waveguides_one_system():
+@pytest.mark.parametrize("segmented", [True, False])
+def test_sixteen_uneven_waveguides_one_system(segmented):
     """The same 16 meshes as one block-diagonal device system (one launch per
-    Krylov kernel for all of them): every mesh's S against the fixture, the
-    per-mesh residual bound <= 1e-8, and the union's solution restricted to
-    a mesh against that mesh's own system (oracle matrices)."""
+    Krylov kernel for all of them), with per-mesh Krylov scalars (row
+    segments) or scalars global over the union: every mesh's S against the
+    fixture, the per-mesh residual bound <= 1e-8, and the union's solution
+    restricted to a mesh against that mesh's own system (oracle matrices)."""
     from emerge_amd import workloads as w
     from emerge_amd.family import FamilySweep
     from oracle import curlcurl as cc
     from oracle import fem
     from oracle import ports as oports
     probs = w.waveguide_family()
-    fam = FamilySweep(probs, [w.waveguide_bcs(p) for p in probs])
+    fam = FamilySweep(probs, [w.waveguide_bcs(p) for p in probs], segmented=segmented)
+    if segmented:
+        rows = fam.sweep.segment_rows
+        assert rows is not None and rows.shape == (len(probs) + 1,) and rows[-1] == fam.sweep.n_red
     freqs = G["family_freqs"]
+    print(f"family union (segmented={segmented}): N={fam.basis.n_field}", flush=True)
     S, bound, res = fam.solve(freqs, precision="mixed", fblock=4)
-    print(f"family union: N={fam.basis.n_field} iters={res.iters.ravel().tolist()} bound={bound.max():.2e}",
-          flush=True)
+    print(f"family union: iters={res.iters.ravel().tolist()} bound={bound.max():.2e}", flush=True)
     assert np.all(bound <= 1e-8), bound.max()
     worst = max(float(np.abs(S[m] - G["family_S"][m]).max()) for m in range(len(probs)))
     print(f"family union: max |dS| {worst:.2e}", flush=True)
