@@ -926,7 +926,17 @@ struct CgState {
     const int* sblk = nullptr;      // [S + 1]
     const int64_t* srow = nullptr;  // [S + 1]
     const int* rseg = nullptr;      // [n] segment of each row (nullptr when S = 1)
+    // [S] every system of the segment has stopped (set by k_cg_beta, cleared
+    // by the first pass of each inner solve): the SpMV, the update and the
+    // correction skip its rows until the next inner solve
+    int* sdone = nullptr;
 };
+
+// The segment's inner solves have all stopped: its rows are skipped.  A
+// stopped system has alpha = 0, so skipping leaves x, r and z as they are.
+__device__ __forceinline__ bool seg_skip(const CgState& st, int s) {
+    return st.nseg > 1 && st.sdone[s] != 0;
+}
 
 // Scalar slot (segment, system) of element k = row F + f.
 __device__ __forceinline__ int seg_slot(const int* rseg, int64_t k, int F) {
@@ -1160,7 +1170,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_spmv(Op<R> op, const R* __res
     const int64_t nw = int64_t(gridDim.x / 8) * W;
     for (int64_t base = r0 + (int64_t(blockIdx.x / 8) * W + wv) * RPW; base < r1; base += nw * RPW) {
         const int64_t row = base + rs;
-        const bool valid = row < r1;
+        const bool valid = row < r1 && !(st.nseg > 1 && st.sdone[st.rseg[row]]);
         const int64_t o = row * F + fo;
         R px = R(0), py = R(0);
         if (own && valid) {
@@ -1288,12 +1298,13 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_alpha(int nblk, CgState st) {
 template <typename R, int F, bool CONJ>
 __global__ __launch_bounds__(kCgThreads) void k_seg_dot(int64_t nq, const cplx<R>* __restrict__ a,
                                                         const cplx<R>* __restrict__ b, CgState st,
-                                                        double2* __restrict__ part) {
+                                                        double2* __restrict__ part, int skip) {
     __shared__ double2 red[kCgThreads];
     const int tid = threadIdx.x;
     const SegRange sr = seg_range(st, nq, F);
     double2 acc = make_double2(0.0, 0.0);
-    for (int64_t k = sr.k0 + tid; k < sr.k1; k += sr.stride) {
+    const int64_t k1 = skip && seg_skip(st, sr.s) ? sr.k0 : sr.k1;  // stopped: zero sums
+    for (int64_t k = sr.k0 + tid; k < k1; k += sr.stride) {
         const double2 u = to_d(a[k]), v = to_d(b[k]);
         if constexpr (CONJ) {
             acc.x += u.x * v.x + u.y * v.y;
@@ -1352,7 +1363,11 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_update(int64_t nq, cplx<R>* _
     const cplx<R> al = init ? cmk<R>(R(0), R(0)) : from_d<R>(st.alpha[sr.s * F + f]);
     double2 rz = make_double2(0.0, 0.0);
     double rr = 0.0;
-    for (int64_t k = sr.k0 + tid; k < sr.k1; k += sr.stride) {
+    // a new inner solve runs every segment; later passes skip stopped ones
+    // (their sums stay zero)
+    if (init && st.nseg > 1 && blockIdx.x == 0 && tid < st.nseg) st.sdone[tid] = 0;
+    const int64_t k1 = !init && seg_skip(st, sr.s) ? sr.k0 : sr.k1;
+    for (int64_t k = sr.k0 + tid; k < k1; k += sr.stride) {
         cplx<R> rv = r[k];
         if (init) {
             dx[k] = cmk<R>(R(0), R(0));
@@ -1428,6 +1443,14 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_beta(int nblk, CgState st, in
         st.done[sf] = dn;
         st.rho[sf] = rho1;
         st.beta[sf] = be;
+    }
+    if (st.nseg > 1) {
+        __shared__ int live;
+        if (tid == 0) live = 0;
+        __syncthreads();
+        if (tid < F && !st.done[sg * F + tid]) atomicOr(&live, 1);
+        __syncthreads();
+        if (tid == 0) st.sdone[sg] = live ? 0 : 1;
     }
 }
 
@@ -1576,7 +1599,8 @@ __global__ __launch_bounds__(kCgThreads) void k_aux_z(int64_t nq, Op<R> op, cons
     const int tid = threadIdx.x;
     double2 rz = make_double2(0.0, 0.0);
     const SegRange sr = seg_range(st, nq, F);
-    for (int64_t k = sr.k0 + tid; k < sr.k1; k += sr.stride) {
+    const int64_t k1 = seg_skip(st, sr.s) ? sr.k0 : sr.k1;  // cleared by the first pass of an inner solve
+    for (int64_t k = sr.k0 + tid; k < k1; k += sr.stride) {
         const int64_t row = k / F;
         const int f = int(k % F);
         const cplx<R> rv = r[k];
@@ -1718,7 +1742,7 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     // streaming grid: gb workgroups over the whole vector, or per segment a
     // share of them proportional to its rows (at least one)
     const unsigned gb0 = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
-    DBuf<int> sblk;
+    DBuf<int> sblk, sdone;
     unsigned gs = gb0;
     const int* rseg = nullptr;
     if (segd) {
@@ -1731,12 +1755,14 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         }
         if (sb[S] > kCgMaxBlocks) return fail(c, EMG_ERR_ARG, "COCG: %d segments need too many workgroups", S);
         EMG_HIP(c, sblk.alloc(S + 1));
+        EMG_HIP(c, sdone.alloc(S));
         EMG_HIP(c, hipMemcpyAsync(sblk.p, sb.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice, s));
         gs = unsigned(sb[S]);
         cs.nseg = S;
         cs.sblk = sblk.p;
         cs.srow = c->seg_row.p;
         cs.rseg = c->seg_rseg.p;
+        cs.sdone = sdone.p;
         rseg = c->seg_rseg.p;
     }
     const unsigned grow = grid_for(n, 4);
@@ -1773,7 +1799,7 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     // a^H b per inner system (per segment and system with segments)
     auto dot64 = [&](const double2* a, const double2* b, double2* out) {
         if (segd) {
-            k_seg_dot<double, F, true><<<gs, kCgThreads, 0, s>>>(nq, a, b, cs, partial.p);
+            k_seg_dot<double, F, true><<<gs, kCgThreads, 0, s>>>(nq, a, b, cs, partial.p, 0);
             k_seg_dot_final<F><<<S, kCgThreads, 0, s>>>(partial.p, cs, out);
         } else {
             k_dots<double, F><<<dim3(gdots, 1), 256, 0, s>>>(nq, nchunk, a, nq, b, partial.p);
@@ -1907,7 +1933,7 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
                     launch_cg_spmv<R, F, R>(c->spmv_gr, ga, s, op, k2d.p, gam.p, pv.p, q.p, cs);
                 // with segments p^T q is summed per segment by its own pass
                 // (the SpMV's workgroups span segments)
-                if (segd) k_seg_dot<R, F, false><<<gs, kCgThreads, 0, s>>>(nq, pv.p, q.p, cs, partial.p);
+                if (segd) k_seg_dot<R, F, false><<<gs, kCgThreads, 0, s>>>(nq, pv.p, q.p, cs, partial.p, 1);
                 k_cg_alpha<F><<<S, kCgThreads, 0, s>>>(int(ga), cs);
                 k_cg_update<R, F><<<gb, kCgThreads, 0, s>>>(nq, pv.p, q.p, dinv.p, r.p, z.p, dx.p, cs, 0, int(aux));
                 precondition();
