@@ -293,57 +293,60 @@ def config_bench(name: str, nf: int, fblock: int, precision: str = "fp64") -> di
                    points=nf, converged=ok, seconds=round(tm, 3), freq_points_per_s=round(ok / tm, 2),
                    analytic_ports_freq_points_per_s=round(nf / ta, 2),
                    max_dS_vs_analytic_ports=float(np.abs(rm.S - ra.S).max()))
-    elif name == "family":
+    elif name in ("family", "family_union"):
+        # family_union: only the one-system leg with per-mesh scalars (for
+        # kernel traces of that leg alone)
         probs = w.waveguide_family()
-        sws = [FrequencySweep(q.mesh, q.basis, q.eps_r, q.inv_mu_r, w.waveguide_bcs(q)) for q in probs]
         freqs = np.linspace(8e9, 12e9, nf)
         kf = dict(kw, keep_fields=False, fblock=32, restart=20000)
-        for sw in sws[:2]:
-            sw.solve(freqs[:2], precision="mixed", **kf)
-        t0 = time.perf_counter()
-        seq = [sw.solve(freqs, precision="mixed", **kf) for sw in sws]
-        ts = time.perf_counter() - t0
-        conc = [None] * len(sws)
+        seq = None
+        if name == "family":
+            sws = [FrequencySweep(q.mesh, q.basis, q.eps_r, q.inv_mu_r, w.waveguide_bcs(q)) for q in probs]
+            for sw in sws[:2]:
+                sw.solve(freqs[:2], precision="mixed", **kf)
+            t0 = time.perf_counter()
+            seq = [sw.solve(freqs, precision="mixed", **kf) for sw in sws]
+            ts = time.perf_counter() - t0
+            conc = [None] * len(sws)
 
-        def run(i):
-            conc[i] = sws[i].solve(freqs, precision="mixed", **kf)
+            def run(i):
+                conc[i] = sws[i].solve(freqs, precision="mixed", **kf)
 
-        th = [threading.Thread(target=run, args=(i,)) for i in range(len(sws))]
-        t0 = time.perf_counter()
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
-        tc = time.perf_counter() - t0
-        ok_s = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in seq)
-        ok_c = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in conc)
-        for sw in sws:
-            sw.ctx.close()
+            th = [threading.Thread(target=run, args=(i,)) for i in range(len(sws))]
+            t0 = time.perf_counter()
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            tc = time.perf_counter() - t0
+            ok_s = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in seq)
+            ok_c = sum(int(np.sum(np.all(r.relres <= 1e-8, axis=1))) for r in conc)
+            for sw in sws:
+                sw.ctx.close()
+            out.update(sequential={"seconds": round(ts, 3), "converged": ok_s, "freq_points_per_s": round(ok_s / ts, 3)},
+                       concurrent={"seconds": round(tc, 3), "converged": ok_c, "freq_points_per_s": round(ok_c / tc, 3)},
+                       max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))),
+                       mean_iters_per_system_single=float(np.mean([r.iters.mean() for r in seq])))
         # the same meshes as one block-diagonal system: one launch per kernel
         # for all meshes, with per-mesh Krylov scalars (row segments) and,
         # for comparison, with scalars global over the union
         from emerge_amd.family import FamilySweep
-        unions = {}
-        for segd in (True, False):
+        for segd in ((True, False) if name == "family" else (True,)):
             fam = FamilySweep(probs, [w.waveguide_bcs(q) for q in probs], segmented=segd)
             fam.solve(freqs[:2], precision="mixed", **kf)
             t0 = time.perf_counter()
             Sf, bound, rf = fam.solve(freqs, precision="mixed", **kf)
             tf = time.perf_counter() - t0
             ok_f = int(np.sum(np.all(bound <= 1e-8, axis=1))) * len(probs)
-            unions["one_system" if segd else "one_system_global_scalars"] = {
-                "seconds": round(tf, 3), "converged_points": ok_f, "freq_points_per_s": round(ok_f / tf, 3),
-                "reduced_dofs": int(fam.sweep.n_red), "max_iters_per_system": int(rf.iters.max()),
-                "mean_iters_per_system": float(rf.iters.mean()),
-                "max_dS_vs_sequential": float(max(np.abs(Sf[m] - seq[m].S).max() for m in range(len(probs))))}
+            leg = {"seconds": round(tf, 3), "converged_points": ok_f, "freq_points_per_s": round(ok_f / tf, 3),
+                   "reduced_dofs": int(fam.sweep.n_red), "max_iters_per_system": int(rf.iters.max()),
+                   "mean_iters_per_system": float(rf.iters.mean())}
+            if seq is not None:
+                leg["max_dS_vs_sequential"] = float(max(np.abs(Sf[m] - seq[m].S).max() for m in range(len(probs))))
+            out["one_system" if segd else "one_system_global_scalars"] = leg
             fam.ctx.close()
         out.update(workload="16 jittered WR-90 sections (Kuhn 11x5xnz), DOFs U[20k,120k] seed 0",
-                   dofs=[int(q.basis.n_field) for q in probs], points_per_mesh=len(freqs),
-                   sequential={"seconds": round(ts, 3), "converged": ok_s, "freq_points_per_s": round(ok_s / ts, 3)},
-                   concurrent={"seconds": round(tc, 3), "converged": ok_c, "freq_points_per_s": round(ok_c / tc, 3)},
-                   max_dS_seq_vs_conc=float(max(np.abs(a.S - b.S).max() for a, b in zip(seq, conc))),
-                   **unions,
-                   mean_iters_per_system_single=float(np.mean([r.iters.mean() for r in seq])))
+                   dofs=[int(q.basis.n_field) for q in probs], points_per_mesh=len(freqs))
     elif name == "farfield":
         out.update(farfield_bench())
     elif name == "element":
@@ -501,7 +504,7 @@ def main() -> int:
     ap.add_argument("--idr-s", type=int, default=4)
     ap.add_argument("--restart", type=int, default=None,
                     help="GMRES dimension / IDR matvecs per cycle / COCG iterations per cycle")
-    ap.add_argument("--config", choices=["cavity", "patch", "family", "farfield", "element", "modal"], default=None,
+    ap.add_argument("--config", choices=["cavity", "patch", "family", "family_union", "farfield", "element", "modal"], default=None,
                     help="run one full-size solve workload instead of the default bench")
     ap.add_argument("--config-points", type=int, default=None, help="frequency points of --config")
     ap.add_argument("--config-precision", default="fp64", choices=["fp64", "mixed", "m32"],
@@ -544,7 +547,7 @@ def main() -> int:
     from emerge_amd.workloads import cube_assembly
 
     if args.config:
-        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "farfield": 0, "element": 0,
+        nf = args.config_points or {"cavity": 1, "patch": 8, "family": 16, "family_union": 16, "farfield": 0, "element": 0,
                                     "modal": 201}[args.config]
         fb = 8 if args.fblock == "auto" else args.fblock
         print(json.dumps(config_bench(args.config, nf, fb if args.config == "patch" else 32, args.config_precision)),

@@ -1344,6 +1344,7 @@ template <typename R, int F>
 __global__ void k_cg_pupd(int64_t nq, const cplx<R>* __restrict__ z, cplx<R>* __restrict__ p, CgState st) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= nq) return;
+    if (st.nseg > 1 && st.sdone[st.rseg[k / F]]) return;  // stopped segment: p is not read again
     const cplx<R> b = from_d<R>(st.beta[seg_slot(st.rseg, k, F)]);
     p[k] = cfma(b, p[k], z[k]);
 }
@@ -1741,7 +1742,11 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     const unsigned gq = grid_for(nq, 256);
     // streaming grid: gb workgroups over the whole vector, or per segment a
     // share of them proportional to its rows (at least one)
-    const unsigned gb0 = unsigned(std::min<int64_t>(kCgUpdBlocks, std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
+    // (one workgroup per CU, or for large systems up to kCgMaxBlocks so that
+    // each thread keeps about 32 elements: more gathers in flight in k_aux_z)
+    const int64_t gwant = std::max<int64_t>(kCgUpdBlocks, nq / (int64_t(kCgThreads) * 32));
+    const int64_t gcap = kCgMaxBlocks - (segd ? S : 0);
+    const unsigned gb0 = unsigned(std::min<int64_t>(std::min(gwant, gcap), std::max<int64_t>(1, (nq + kCgThreads - 1) / kCgThreads)));
     DBuf<int> sblk, sdone;
     unsigned gs = gb0;
     const int* rseg = nullptr;
@@ -1750,7 +1755,7 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
         for (int g = 0; g < S; ++g) {
             const int64_t rows = c->seg_row_h[g + 1] - c->seg_row_h[g];
             const int64_t cap = std::max<int64_t>(1, (rows * F + kCgThreads - 1) / kCgThreads);
-            const int64_t want = std::llround(double(gb0) * double(rows) / double(n));
+            const int64_t want = int64_t(gb0) * rows / n;  // sum <= gb0, so sb[S] <= gb0 + S
             sb[g + 1] = sb[g] + int(std::max<int64_t>(1, std::min(cap, want)));
         }
         if (sb[S] > kCgMaxBlocks) return fail(c, EMG_ERR_ARG, "COCG: %d segments need too many workgroups", S);
