@@ -1772,7 +1772,12 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     }
     const unsigned grow = grid_for(n, 4);
     const unsigned gdots = grid_for(nchunk * 64, 256);
-    const unsigned ga = unsigned(8 * std::min<int64_t>(kCgMaxBlocks / 8, std::max<int64_t>(1, (n + 127) / 128)));
+    // SpMV grid: at most 1,024 workgroups and at most 16,384 partial sums
+    // (ga F), which the one-workgroup alpha reduction reads every iteration
+    // (WR-90 sweep at F = 64: 11.6 us per reduction with 1,024 workgroups;
+    // 349 -> 376 freq-points/s with 256)
+    const int64_t gacap = std::min<int64_t>(kCgMaxBlocks, std::max(8, 16384 / F));
+    const unsigned ga = unsigned(8 * std::min<int64_t>(gacap / 8, std::max<int64_t>(1, (n + 127) / 128)));
     const unsigned gb = gs;
     k_jacobi<R, F><<<gq, 256, 0, s>>>(op, k2d.p, gam.p, dinv.p);
     const unsigned gaux = aux ? grid_for(op.aux_n * F, 256) : 1;
