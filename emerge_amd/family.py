@@ -49,6 +49,23 @@ def _shifted_bc(b, dx: float, tri_off: int, port_offset: int):
     raise TypeError(f"unsupported boundary condition {type(b).__name__}")
 
 
+def dof_segments(mesh: Mesh3D, basis: Nedelec2, vert_off) -> np.ndarray:
+    """Mesh index of every field DOF of a union mesh whose meshes own the
+    vertex ranges [vert_off[m], vert_off[m+1]): the mesh of the DOF's edge or
+    triangle (all its vertices lie in one mesh)."""
+    voff = np.asarray(vert_off, dtype=np.int64)
+    seg = np.full(basis.n_field, -1, dtype=np.int64)
+    e_mesh = np.searchsorted(voff, mesh.edges[0], side="right") - 1
+    t_mesh = np.searchsorted(voff, mesh.tris[0], side="right") - 1
+    if np.any(e_mesh != np.searchsorted(voff, mesh.edges[1], side="right") - 1):
+        raise ValueError("an edge joins two meshes")
+    seg[basis.edge_to_field] = e_mesh[None, :]
+    seg[basis.tri_to_field] = t_mesh[None, :]
+    if np.any(seg < 0):
+        raise ValueError("a field DOF without a mesh")
+    return seg
+
+
 class FamilySweep:
     """A family of independent meshes (each with isotropic scalar materials
     and its own boundary conditions, the same number of ports each) solved as
@@ -99,16 +116,7 @@ class FamilySweep:
         self.segmented = bool(segmented)
         seg = None
         if self.segmented:
-            # mesh of every field DOF, from the mesh of its edge or triangle
-            ub = self.basis
-            voff = np.asarray(self.vert_off, dtype=np.int64)
-            seg = np.full(ub.n_field, -1, dtype=np.int64)
-            e_mesh = np.searchsorted(voff, self.mesh.edges[0], side="right") - 1
-            t_mesh = np.searchsorted(voff, self.mesh.tris[0], side="right") - 1
-            seg[ub.edge_to_field] = e_mesh[None, :]
-            seg[ub.tri_to_field] = t_mesh[None, :]
-            if np.any(seg < 0):
-                raise AssertionError("a field DOF without a mesh")
+            seg = dof_segments(self.mesh, self.basis, self.vert_off)
         self.sweep = FrequencySweep(self.mesh, self.basis, np.concatenate(eps), np.concatenate(inu), all_bcs,
                                     device=device, dof_segments=seg)
         P = self.n_ports

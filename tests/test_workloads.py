@@ -47,3 +47,32 @@ def test_waveguide_family_sizes():
     for p in fam:
         assert p.basis.n_field >= p.meta["target_dofs"]
         assert p.basis.n_field < p.meta["target_dofs"] + 3000
+
+
+def test_family_dof_segments():
+    """The mesh of every field DOF of a union of independent meshes: each
+    tet's 20 DOFs lie in the tet's own mesh, and each mesh keeps exactly its
+    own number of DOFs (FamilySweep's row segments)."""
+    from emerge_amd import workloads as w
+    from emerge_amd.basis import Nedelec2
+    from emerge_amd.family import dof_segments
+    from emerge_amd.mesh import Mesh3D
+    parts = [w.waveguide(cells=(3, 2, k), length=5e-3 * k) for k in (2, 4, 3)]
+    nodes, tets, voff, tet_mesh = [], [], [], []
+    v = 0
+    x0 = 0.0
+    for m, p in enumerate(parts):
+        xyz = p.mesh.nodes.copy()
+        xyz[0] += x0 - xyz[0].min()
+        x0 = xyz[0].max() + 1e-3
+        nodes.append(xyz)
+        tets.append(p.mesh.tets + v)
+        voff.append(v)
+        tet_mesh.append(np.full(p.mesh.tets.shape[1], m))
+        v += p.mesh.n_nodes
+    union = Mesh3D(np.concatenate(nodes, axis=1), np.concatenate(tets, axis=1))
+    basis = Nedelec2(union)
+    seg = dof_segments(union, basis, voff)
+    tm = np.concatenate(tet_mesh)
+    assert np.all(seg[basis.tet_to_field] == tm[None, :])
+    assert np.array_equal(np.bincount(seg, minlength=len(parts)), [p.basis.n_field for p in parts])
