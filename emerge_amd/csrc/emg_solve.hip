@@ -1679,7 +1679,7 @@ int run_cocg(emg_ctx* c, const Op<R>& op, const Op<double>& op64, const double* 
     const int64_t n = op.n, nq = n * F;
     // row segments (emg_solver_set_segments) of this system: S sets of
     // Krylov scalars per batch, NS = S F inner systems
-    const bool segd = c->seg_nseg > 1 && c->seg_n == n;
+    const bool segd = c->seg_nseg > 1 && c->seg_n == n && op.ptr == c->red_ptr.p;  // the sweep's system only
     const int S = segd ? c->seg_nseg : 1;
     const int NS = S * F;
     DBuf<cplx<R>> pv, q, r, z, dx, dinv;

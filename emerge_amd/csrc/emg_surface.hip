@@ -408,6 +408,10 @@ extern "C" int emg_system_reduce(emg_ctx* c, int64_t n_keep, const int32_t* keep
             return fail(c, EMG_ERR_ARG, "emg_system_reduce: keep ids must be distinct and in [0, N)");
         keep[r] = int32_t(k);
     }
+    // row segments belong to the previous row order
+    c->seg_nseg = 0;
+    c->seg_n = 0;
+    c->seg_row_h.clear();
     hipStream_t st = c->stream;
     const int64_t N = c->n_field;
     EMG_HIP(c, c->keep.alloc(N));

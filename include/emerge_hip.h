@@ -258,8 +258,9 @@ int emg_solver_set_spmv_lanes(emg_ctx* ctx, int entries_in_flight);
  * own Krylov recurrence (its own scalars, stop test and refinement on its
  * own right-hand side) inside the same kernel launches, so every mesh
  * converges as it would alone; relres and iterations are reported per system
- * as those of its slowest segment.  nseg <= 1 clears them.  At most 256
- * segments; COCG only.  Replaces the reference's one solve per job
+ * as those of its slowest segment.  nseg <= 1 clears them, and so does
+ * emg_system_reduce; they apply to the reduced system only (not to
+ * emg_csr_solve).  At most 256 segments; COCG only.  Replaces the reference's one solve per job
  * (MW/microwave_3d.py:773-861, sequential, threaded or process-pool). */
 int emg_solver_set_segments(emg_ctx* ctx, int32_t nseg, const int64_t* row_start);
 
