@@ -1272,7 +1272,7 @@ void launch_cg_spmv(int gr, unsigned grid, hipStream_t s, const Op<R>& op, const
 }
 
 // alpha = rho / p^T q (one workgroup per segment; partials of k_cg_spmv,
-// or of k_cg_pq with several segments)
+// or of k_seg_dot with several segments)
 template <int F>
 __global__ __launch_bounds__(kCgThreads) void k_cg_alpha(int nblk, CgState st) {
     __shared__ double2 red[kCgThreads];
